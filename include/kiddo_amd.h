@@ -1,0 +1,166 @@
+/* kiddo_amd.h — C ABI of the MI355X batch query engine for Kiddo's ImmutableKdTree.
+ *
+ * The reference exposes one query at a time through a fluent builder
+ * (kiddo src/kd_tree/query/builder.rs:2318-2331 `KdTree::query`, families at :2008-2077,
+ * `execute` at :1693); callers batch by looping or with rayon
+ * (examples/immutable-large.rs:59-69). Each entry point below is that builder chain run
+ * over a whole batch of queries on one GPU. Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns KD_OK (0) or a negative KD_ERR_* code and never aborts;
+ *     kd_last_error() describes the last failure on the calling thread.
+ *   - Query and result pointers are DEVICE pointers (HBM); `stream` is a hipStream_t
+ *     (NULL = the default stream). Calls are stream-ordered and asynchronous except where
+ *     noted; results are ready once the stream is synchronised.
+ *   - dtype: KD_F32 / KD_F64 / KD_U16 / KD_U32 coordinates (the tree's A). Queries are a
+ *     row-major m x dims array of A (Kiddo's `&[A; K]` per query).
+ *   - Distances are SquaredEuclidean in the output type O: O = A for f32/f64 trees, and
+ *     O = f64 for u16/u32 trees (exact there: every distance is an integer below 2^53).
+ *   - Radii are in metric units, i.e. SQUARED distance (Kiddo's `within(radius)` argument
+ *     is `Dq::Output`, builder.rs:2033). Pass +INFINITY for "no radius".
+ *   - Result order: by (distance, then lower item) for nearest_n and sorted within;
+ *     by item ascending for best_n_within; Kiddo's boundary rules (inclusive `<=` unless
+ *     KD_EXCLUSIVE, builder.rs:1271-1278).
+ */
+#ifndef KIDDO_AMD_H
+#define KIDDO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum kd_status {
+  KD_OK = 0,
+  KD_ERR_INVALID = -1,        /* bad argument (null, size, NaN coordinate, ...)          */
+  KD_ERR_UNSUPPORTED = -2,    /* dtype/dims/flag combination not provided                */
+  KD_ERR_DEVICE = -3,         /* HIP runtime error                                       */
+  KD_ERR_LEAF_OVERFLOW = -4,  /* a leaf would exceed 1024 entries (construction.rs:526)  */
+  KD_ERR_TOO_DEEP = -5,       /* more stem levels than the engine supports               */
+  KD_ERR_NO_MEMORY = -6,      /* device or host allocation failed                        */
+  KD_ERR_CAPACITY = -7        /* caller-provided output capacity too small               */
+};
+
+enum kd_dtype { KD_F32 = 0, KD_F64 = 1, KD_U16 = 2, KD_U32 = 3 };
+
+/* query flags */
+enum kd_query_flags {
+  KD_EXCLUSIVE = 1u << 0, /* `.exclusive_boundaries()`: strict `< radius`              */
+  KD_UNSORTED = 1u << 1,  /* `.unsorted()`: within results in visit order              */
+  KD_MORTON = 1u << 2,    /* traverse queries in Morton order (results keep input order) */
+  KD_SCREEN_F32 = 1u << 3 /* f64 trees: FP32 screen + FP64 rescore (same results)      */
+};
+
+/* tree-create flags */
+enum kd_tree_flags {
+  KD_TREE_SHADOW_F32 = 1u << 0 /* f64 trees: keep an FP32 shadow of the leaves            */
+};
+
+/* A tree in Kiddo's ImmutableKdTree layout, exactly what a Rust shim reads through the
+ * public KdTreeAccessor (kiddo src/traits/kd_tree.rs:16-46) and
+ * VecOfArenas (src/leaf_strategy/vec_of_arenas.rs:60-69). All pointers are host memory. */
+typedef struct kd_tree_desc {
+  int32_t dtype;            /* kd_dtype                                                */
+  int32_t dims;             /* K (2, 3 or 4)                                           */
+  int32_t bucket;           /* B (construction bucket size)                            */
+  int32_t max_stem_level;   /* KdTree::max_stem_level; -1 for a stemless tree          */
+  uint64_t size;            /* number of entries                                       */
+  uint64_t stems_len;       /* number of A values in `stems`                           */
+  const void* stems;        /* Eytzinger stems, index 0 unused, sentinel = A::MAX/+inf */
+  uint64_t leaf_count;      /* 2^(max_stem_level+1), or 1/0 for stemless trees         */
+  const uint64_t* leaf_extents; /* leaf_count pairs (byte offset, entry count)         */
+  uint64_t leaf_bytes_len;
+  const uint8_t* leaf_bytes;    /* VecOfArenas arena (tiles 32/8/4/2/1, coords then u32 items) */
+} kd_tree_desc;
+
+typedef struct kd_layout kd_layout; /* host-side tree produced by kd_build_layout */
+typedef struct kd_tree kd_tree;     /* device-resident tree                         */
+
+typedef struct kd_tree_info {
+  int32_t dtype, dims, max_stem_level, has_shadow_f32;
+  uint64_t size, leaf_count, max_leaf_len;
+  uint64_t device_bytes;    /* HBM held by the tree */
+  double bbox_min[4], bbox_max[4];
+} kd_tree_info;
+
+/* ----------------------------------------------------------------------- misc ---- */
+const char* kd_last_error(void);
+int kd_version(void);
+
+/* ------------------------------------------------------------ host construction ---- */
+/* Builds the ImmutableKdTree layout on the host, as ImmutableKdTree::new_from_slice
+ * (items == NULL: item = source index) or new_from_entries (explicit u32 items)
+ * (kiddo src/kd_tree/construction.rs:278-303 and :403-411). points: row-major n x dims.
+ * threads <= 0: all host cores. */
+int kd_build_layout(int32_t dtype, int32_t dims, int32_t bucket, uint64_t n,
+                    const void* points, const uint32_t* items, int32_t threads,
+                    kd_layout** out);
+/* Describes a built layout (pointers stay valid until kd_layout_free). */
+int kd_layout_desc(const kd_layout* layout, kd_tree_desc* out);
+void kd_layout_free(kd_layout* layout);
+
+/* ----------------------------------------------------------------- device trees ---- */
+/* Uploads a Kiddo-layout tree to the current HIP device and re-lays it out for the
+ * kernels (synchronous). Replaces holding `&ImmutableKdTree` (kiddo src/lib.rs:270-271).
+ * Rejects NaN coordinates, leaves over 1024 entries and inconsistent extents. */
+int kd_tree_create(const kd_tree_desc* desc, uint32_t tree_flags, kd_tree** out);
+int kd_tree_destroy(kd_tree* tree);
+int kd_tree_get_info(const kd_tree* tree, kd_tree_info* out);
+
+/* --------------------------------------------------------------------- queries ---- */
+/* tree.query(q).nearest_one::<SquaredEuclidean>().execute()  (builder.rs:2008-2017;
+ * query/nearest_one.rs:61-143). items[m], dists[m] (O). Empty tree: (+inf, 0). */
+int kd_nearest_one(kd_tree* tree, const void* queries, uint64_t m, uint32_t flags,
+                   uint32_t* items, void* dists, void* stream);
+
+/* tree.query(q).nearest_one().approx().execute()  (builder.rs:2059-2065;
+ * query/approx_nearest_one.rs:49-71): scans only the leaf the query descends to.
+ * Float trees only (Kiddo requires Output == A). dists are in A. */
+int kd_approx_nearest_one(kd_tree* tree, const void* queries, uint64_t m, uint32_t flags,
+                          uint32_t* items, void* dists, void* stream);
+
+/* tree.query(q).nearest_n(k)[.within(radius)][.exclusive_boundaries()][.unsorted()]
+ * (builder.rs:2019-2042; query/nearest_n.rs:21-38, nearest_n_within.rs:127-179).
+ * radius = +INFINITY for plain nearest_n. Row-major m x k outputs; row i holds counts[i]
+ * results sorted by (distance, item) and is padded with item 0xFFFFFFFF, distance +inf.
+ * KD_SCREEN_F32 (f64 trees created with KD_TREE_SHADOW_F32) screens in FP32 against a
+ * certified error bound and rescores survivors in FP64: results are identical. */
+int kd_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k, double radius,
+                 uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream);
+
+/* tree.query(q).within(radius)[.unsorted()][.exclusive_boundaries()], phase 1
+ * (query/within.rs:18-41, within_unsorted.rs:133-156): counts results per query and
+ * writes offsets[m+1] as their exclusive prefix sum (offsets[m] = total). */
+int kd_within_count(kd_tree* tree, const void* queries, uint64_t m, double radius,
+                    uint32_t flags, uint64_t* offsets, void* stream);
+/* Phase 2: writes each query's results into items/dists[offsets[i] .. offsets[i+1]).
+ * Sorted by (distance, item) unless KD_UNSORTED (then traversal visit order). */
+int kd_within_fill(kd_tree* tree, const void* queries, uint64_t m, double radius,
+                   uint32_t flags, const uint64_t* offsets, uint32_t* items, void* dists,
+                   void* stream);
+
+/* tree.query(q).best_n_within(radius, n)[.exclusive_boundaries()]  (builder.rs:2044-2057;
+ * query/best_n_within.rs:88-196): up to n entries within radius with the smallest items.
+ * Row-major m x n outputs sorted by item ascending, padded like kd_nearest_n. */
+int kd_best_n_within(kd_tree* tree, const void* queries, uint64_t m, double radius,
+                     uint32_t n, uint32_t flags, uint32_t* items, void* dists,
+                     uint32_t* counts, void* stream);
+
+/* Traversal counters of the last query call on this tree (synchronous; calls that
+ * overlap on one tree share them): out4 = {leaves visited, entries scanned, stem nodes
+ * read, entries re-measured in FP64 by KD_SCREEN_F32}. The first three equal what Kiddo's
+ * DFS visits for the same queries (its leaf-visit hook: query/nearest_one.rs:31-32). */
+int kd_query_stats(kd_tree* tree, uint64_t* out4);
+
+/* Timing of the main query kernel of each call with HIP events recorded on the call's
+ * stream (no Kiddo counterpart: measurement support). kd_last_kernel_ms waits for the
+ * last timed call and returns its kernel's elapsed milliseconds. */
+int kd_tree_set_timing(kd_tree* tree, int32_t enable);
+int kd_last_kernel_ms(kd_tree* tree, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KIDDO_AMD_H */

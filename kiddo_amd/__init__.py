@@ -1,0 +1,452 @@
+"""kiddo_amd — MI355X batch query engine for Kiddo's ImmutableKdTree.
+
+Python mirror of Kiddo's query builder (kiddo src/kd_tree/query/builder.rs:2318-2331
+``KdTree::query`` and the families at :2008-2077) over the C ABI in include/kiddo_amd.h.
+Where Kiddo answers one query per ``execute()``, ``tree.query(queries)`` here takes a
+whole batch (an m x K array) and runs it on the GPU:
+
+    tree = ImmutableKdTree.new_from_slice(points)                 # host build + upload
+    r = tree.query(q).nearest_n(16).execute()                     # r.items, r.distances
+    r = tree.query(q).nearest_n(8).within(0.01).exclusive_boundaries().execute()
+    r = tree.query(q).nearest_one().execute()                     # r.item, r.distance
+    r = tree.query(q).nearest_one().approx().execute()
+    r = tree.query(q).within(0.01).unsorted().execute()           # CSR: r.offsets
+    r = tree.query(q).best_n_within(0.01, 5).execute()
+
+Radii are squared distances (SquaredEuclidean output units, like Kiddo's ``within``).
+torch provides device memory and streams only; every query runs in the HIP kernels of
+libkiddo_amd.so, and importing this module fails when that library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkiddo_amd.so")
+
+KD_OK = 0
+KD_F32, KD_F64, KD_U16, KD_U32 = 0, 1, 2, 3
+KD_EXCLUSIVE = 1 << 0
+KD_UNSORTED = 1 << 1
+KD_MORTON = 1 << 2
+KD_SCREEN_F32 = 1 << 3
+KD_TREE_SHADOW_F32 = 1 << 0
+
+_NP_CODE = {np.dtype(np.float32): KD_F32, np.dtype(np.float64): KD_F64,
+            np.dtype(np.uint16): KD_U16, np.dtype(np.uint32): KD_U32}
+_A_NP = {KD_F32: np.float32, KD_F64: np.float64, KD_U16: np.uint16, KD_U32: np.uint32}
+_O_NP = {KD_F32: np.float32, KD_F64: np.float64, KD_U16: np.float64, KD_U32: np.float64}
+
+
+class KiddoAmdError(RuntimeError):
+    pass
+
+
+class TreeDesc(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("dims", C.c_int32), ("bucket", C.c_int32),
+                ("max_stem_level", C.c_int32), ("size", C.c_uint64),
+                ("stems_len", C.c_uint64), ("stems", C.c_void_p),
+                ("leaf_count", C.c_uint64), ("leaf_extents", C.c_void_p),
+                ("leaf_bytes_len", C.c_uint64), ("leaf_bytes", C.c_void_p)]
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("dims", C.c_int32), ("max_stem_level", C.c_int32),
+                ("has_shadow_f32", C.c_int32), ("size", C.c_uint64),
+                ("leaf_count", C.c_uint64), ("max_leaf_len", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("bbox_min", C.c_double * 4),
+                ("bbox_max", C.c_double * 4)]
+
+
+# The exported entry points and their C signatures (include/kiddo_amd.h).
+_vp, _u64, _u32, _i32, _f64 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32, C.c_double
+SIGNATURES = {
+    "kd_last_error": ([], C.c_char_p),
+    "kd_version": ([], C.c_int),
+    "kd_build_layout": ([_i32, _i32, _i32, _u64, _vp, _vp, _i32, C.POINTER(_vp)], C.c_int),
+    "kd_layout_desc": ([_vp, C.POINTER(TreeDesc)], C.c_int),
+    "kd_layout_free": ([_vp], None),
+    "kd_tree_create": ([C.POINTER(TreeDesc), _u32, C.POINTER(_vp)], C.c_int),
+    "kd_tree_destroy": ([_vp], C.c_int),
+    "kd_tree_get_info": ([_vp, C.POINTER(TreeInfo)], C.c_int),
+    "kd_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
+    "kd_approx_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
+    "kd_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_within_count": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp], C.c_int),
+    "kd_within_fill": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_best_n_within": ([_vp, _vp, _u64, _f64, _u32, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_query_stats": ([_vp, _vp], C.c_int),
+    "kd_tree_set_timing": ([_vp, _i32], C.c_int),
+    "kd_last_kernel_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libkiddo_amd.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KiddoAmdError(
+                f"{LIB_PATH} is missing: build it with `python -m kiddo_amd._build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != KD_OK:
+        msg = lib().kd_last_error().decode(errors="replace")
+        raise KiddoAmdError(f"{what} failed ({rc}): {msg}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _np_ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+# ------------------------------------------------------------------ host layout ------
+
+class HostLayout:
+    """Kiddo-layout tree built on the host (ImmutableKdTree::new_from_slice /
+    new_from_entries, construction.rs:278-303, :403-411)."""
+
+    def __init__(self, points: np.ndarray, bucket: int = 32, items=None, threads: int = 0):
+        pts = np.ascontiguousarray(points)
+        if pts.ndim != 2:
+            raise ValueError("points must be an (n, K) array")
+        self.dtype_code = _NP_CODE[pts.dtype]
+        self.n, self.K = pts.shape
+        self.bucket = bucket
+        it = None if items is None else np.ascontiguousarray(items, dtype=np.uint32)
+        h = C.c_void_p()
+        _check(lib().kd_build_layout(self.dtype_code, self.K, bucket, self.n, _np_ptr(pts),
+                                     _np_ptr(it), threads, C.byref(h)), "kd_build_layout")
+        self._h = h
+        self.desc = TreeDesc()
+        _check(lib().kd_layout_desc(self._h, C.byref(self.desc)), "kd_layout_desc")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().kd_layout_free(self._h)
+            self._h = None
+
+    @property
+    def max_stem_level(self):
+        return self.desc.max_stem_level
+
+    def stems(self) -> np.ndarray:
+        a = np.dtype(_A_NP[self.dtype_code])
+        n = self.desc.stems_len
+        if n == 0:
+            return np.zeros(0, a)
+        buf = (C.c_uint8 * (n * a.itemsize)).from_address(self.desc.stems)
+        return np.frombuffer(buf, a).copy()
+
+    def extents(self) -> np.ndarray:
+        n = self.desc.leaf_count
+        if n == 0:
+            return np.zeros((0, 2), np.uint64)
+        buf = (C.c_uint64 * (2 * n)).from_address(self.desc.leaf_extents)
+        return np.frombuffer(buf, np.uint64).reshape(-1, 2).copy()
+
+    def arena(self) -> np.ndarray:
+        n = self.desc.leaf_bytes_len
+        if n == 0:
+            return np.zeros(0, np.uint8)
+        buf = (C.c_uint8 * n).from_address(self.desc.leaf_bytes)
+        return np.frombuffer(buf, np.uint8).copy()
+
+
+# ------------------------------------------------------------------ results ----------
+
+@dataclass
+class NearestOne:
+    item: object       # (m,) uint32
+    distance: object   # (m,) O
+
+
+@dataclass
+class NearestN:
+    items: object      # (m, k) uint32, padded with 0xFFFFFFFF
+    distances: object  # (m, k) O, padded with +inf
+    counts: object     # (m,) uint32
+
+
+@dataclass
+class Within:
+    offsets: object    # (m + 1,) int64 CSR offsets
+    items: object
+    distances: object
+
+
+# ------------------------------------------------------------------ device tree ------
+
+class ImmutableKdTree:
+    """A Kiddo ImmutableKdTree<A, K> resident in HBM (kiddo src/lib.rs:270-271)."""
+
+    def __init__(self, desc: TreeDesc, shadow_f32: bool = False, keepalive=None):
+        h = C.c_void_p()
+        flags = KD_TREE_SHADOW_F32 if shadow_f32 else 0
+        _check(lib().kd_tree_create(C.byref(desc), flags, C.byref(h)), "kd_tree_create")
+        self._h = h
+        self._keep = keepalive
+        info = TreeInfo()
+        _check(lib().kd_tree_get_info(self._h, C.byref(info)), "kd_tree_get_info")
+        self.info = info
+        self.dtype_code = info.dtype
+        self.K = info.dims
+        self.size = info.size
+
+    @classmethod
+    def new_from_slice(cls, points: np.ndarray, bucket: int = 32, shadow_f32: bool = False,
+                       threads: int = 0):
+        """ImmutableKdTree::new_from_slice: items are the source indices."""
+        lay = HostLayout(points, bucket, None, threads)
+        return cls(lay.desc, shadow_f32, keepalive=None)
+
+    @classmethod
+    def new_from_entries(cls, points: np.ndarray, items, bucket: int = 32,
+                         shadow_f32: bool = False, threads: int = 0):
+        lay = HostLayout(points, bucket, items, threads)
+        return cls(lay.desc, shadow_f32, keepalive=None)
+
+    @classmethod
+    def from_layout(cls, dtype, K, bucket, size, max_stem_level, stems, extents, arena,
+                    shadow_f32: bool = False):
+        """Uploads a tree given in Kiddo's own buffers (stems, leaf extents, leaf arena)."""
+        stems = np.ascontiguousarray(stems)
+        extents = np.ascontiguousarray(extents, dtype=np.uint64)
+        arena = np.ascontiguousarray(arena, dtype=np.uint8)
+        d = TreeDesc()
+        d.dtype = _NP_CODE[np.dtype(dtype)]
+        d.dims = K
+        d.bucket = bucket
+        d.max_stem_level = max_stem_level
+        d.size = size
+        d.stems_len = stems.size
+        d.stems = stems.ctypes.data if stems.size else None
+        d.leaf_count = extents.size // 2
+        d.leaf_extents = extents.ctypes.data if extents.size else None
+        d.leaf_bytes_len = arena.size
+        d.leaf_bytes = arena.ctypes.data if arena.size else None
+        return cls(d, shadow_f32, keepalive=(stems, extents, arena))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kd_tree_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def a_dtype(self):
+        return _A_NP[self.dtype_code]
+
+    @property
+    def out_dtype(self):
+        return _O_NP[self.dtype_code]
+
+    def query(self, queries, stream=None) -> "QueryBuilder":
+        return QueryBuilder(self, queries, stream)
+
+    def query_stats(self):
+        """{leaves, points, stems, rescored} of the last query call on this tree."""
+        out = np.zeros(4, np.uint64)
+        _check(lib().kd_query_stats(self._h, _np_ptr(out)), "kd_query_stats")
+        return dict(zip(("leaves", "points", "stems", "rescored"), (int(x) for x in out)))
+
+    def set_timing(self, enable: bool = True):
+        _check(lib().kd_tree_set_timing(self._h, int(enable)), "kd_tree_set_timing")
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        _check(lib().kd_last_kernel_ms(self._h, C.byref(ms)), "kd_last_kernel_ms")
+        return ms.value
+
+    def screen_stats(self):
+        st = self.query_stats()
+        return st["points"], st["rescored"]
+
+    # -- raw batch calls on device tensors (used by the builder, benches and tests) ------
+    def _queries(self, q):
+        torch = _torch()
+        tdt = {np.float32: torch.float32, np.float64: torch.float64,
+               np.uint16: torch.uint16, np.uint32: torch.uint32}[self.a_dtype]
+        if isinstance(q, np.ndarray):
+            q = torch.from_numpy(np.ascontiguousarray(q, dtype=self.a_dtype)).to("cuda")
+        if q.dtype != tdt or q.dim() != 2 or q.shape[1] != self.K or not q.is_cuda:
+            raise ValueError(f"queries must be a cuda ({tdt}) tensor of shape (m, {self.K})")
+        return q.contiguous()
+
+    def _out(self, *shape, dtype=None):
+        torch = _torch()
+        dt = dtype or {np.float32: torch.float32, np.float64: torch.float64}[self.out_dtype]
+        return torch.empty(shape, dtype=dt, device="cuda")
+
+    def nearest_one_raw(self, q, flags=0, stream=None, approx=False):
+        torch = _torch()
+        q = self._queries(q)
+        m = q.shape[0]
+        items = torch.empty(m, dtype=torch.uint32, device="cuda")
+        if approx:
+            d = self._out(m, dtype={np.float32: torch.float32, np.float64: torch.float64}[self.a_dtype])
+            fn, name = lib().kd_approx_nearest_one, "kd_approx_nearest_one"
+        else:
+            d = self._out(m)
+            fn, name = lib().kd_nearest_one, "kd_nearest_one"
+        _check(fn(self._h, q.data_ptr(), m, flags, items.data_ptr(), d.data_ptr(),
+                  _stream_ptr(stream)), name)
+        return NearestOne(items, d)
+
+    def nearest_n_raw(self, q, k, radius=math.inf, flags=0, stream=None, out=None):
+        torch = _torch()
+        q = self._queries(q)
+        m = q.shape[0]
+        if out is None:
+            out = NearestN(torch.empty((m, k), dtype=torch.uint32, device="cuda"),
+                           self._out(m, k), torch.empty(m, dtype=torch.uint32, device="cuda"))
+        _check(lib().kd_nearest_n(self._h, q.data_ptr(), m, k, float(radius), flags,
+                                  out.items.data_ptr(), out.distances.data_ptr(),
+                                  out.counts.data_ptr(), _stream_ptr(stream)), "kd_nearest_n")
+        return out
+
+    def within_raw(self, q, radius, flags=0, stream=None):
+        torch = _torch()
+        q = self._queries(q)
+        m = q.shape[0]
+        offsets = torch.empty(m + 1, dtype=torch.int64, device="cuda")
+        sp = _stream_ptr(stream)
+        _check(lib().kd_within_count(self._h, q.data_ptr(), m, float(radius), flags,
+                                     offsets.data_ptr(), sp), "kd_within_count")
+        total = int(offsets[m].item()) if m else 0
+        items = torch.empty(max(total, 1), dtype=torch.uint32, device="cuda")
+        d = self._out(max(total, 1))
+        _check(lib().kd_within_fill(self._h, q.data_ptr(), m, float(radius), flags,
+                                    offsets.data_ptr(), items.data_ptr(), d.data_ptr(), sp),
+               "kd_within_fill")
+        return Within(offsets, items[:total], d[:total])
+
+    def best_n_within_raw(self, q, radius, n, flags=0, stream=None):
+        torch = _torch()
+        q = self._queries(q)
+        m = q.shape[0]
+        out = NearestN(torch.empty((m, n), dtype=torch.uint32, device="cuda"),
+                       self._out(m, n), torch.empty(m, dtype=torch.uint32, device="cuda"))
+        _check(lib().kd_best_n_within(self._h, q.data_ptr(), m, float(radius), n, flags,
+                                      out.items.data_ptr(), out.distances.data_ptr(),
+                                      out.counts.data_ptr(), _stream_ptr(stream)),
+               "kd_best_n_within")
+        return out
+
+
+class QueryBuilder:
+    """Fluent batch builder mirroring kiddo src/kd_tree/query/builder.rs."""
+
+    def __init__(self, tree: ImmutableKdTree, queries, stream=None):
+        self.tree = tree
+        self.queries = queries
+        self.stream = stream
+        self.family = None
+        self.k = None
+        self.radius = math.inf
+        self.flags = 0
+        self.is_approx = False
+
+    def _family(self, fam):
+        if self.family is not None:
+            raise ValueError("query family already chosen")
+        self.family = fam
+        return self
+
+    # families (builder.rs:2008-2057)
+    def nearest_one(self):
+        return self._family("nearest_one")
+
+    def nearest_n(self, k: int):
+        if k < 1:
+            raise ValueError("max_qty must be non-zero")
+        self.k = k
+        return self._family("nearest_n")
+
+    def within(self, radius: float):
+        if self.family == "nearest_n":  # nearest_n(k).within(radius)
+            self.radius = radius
+            return self
+        self.radius = radius
+        return self._family("within")
+
+    def best_n_within(self, radius: float, n: int):
+        if n < 1:
+            raise ValueError("max_qty must be non-zero")
+        self.radius = radius
+        self.k = n
+        return self._family("best_n_within")
+
+    # options (builder.rs:2059-2081)
+    def approx(self):
+        if self.family != "nearest_one":
+            raise ValueError("approx() is only available after nearest_one()")
+        self.is_approx = True
+        return self
+
+    def unsorted(self):
+        if self.family not in ("within", "nearest_n"):
+            raise ValueError("unsorted() applies to within / nearest_n")
+        self.flags |= KD_UNSORTED
+        return self
+
+    def exclusive_boundaries(self):
+        if self.family not in ("within", "nearest_n", "best_n_within"):
+            raise ValueError("exclusive_boundaries() applies to radius queries")
+        self.flags |= KD_EXCLUSIVE
+        return self
+
+    # engine options (no Kiddo counterpart: they change speed, not results)
+    def morton(self):
+        self.flags |= KD_MORTON
+        return self
+
+    def screen_f32(self):
+        self.flags |= KD_SCREEN_F32
+        return self
+
+    def execute(self):
+        t = self.tree
+        if self.family == "nearest_one":
+            return t.nearest_one_raw(self.queries, self.flags, self.stream, approx=self.is_approx)
+        if self.family == "nearest_n":
+            return t.nearest_n_raw(self.queries, self.k, self.radius, self.flags, self.stream)
+        if self.family == "within":
+            return t.within_raw(self.queries, self.radius, self.flags, self.stream)
+        if self.family == "best_n_within":
+            return t.best_n_within_raw(self.queries, self.radius, self.k, self.flags, self.stream)
+        raise ValueError("choose a query family first")
+
+
+__all__ = ["ImmutableKdTree", "HostLayout", "QueryBuilder", "NearestOne", "NearestN",
+           "Within", "KiddoAmdError", "lib", "SIGNATURES", "KD_EXCLUSIVE", "KD_UNSORTED",
+           "KD_MORTON", "KD_SCREEN_F32", "KD_TREE_SHADOW_F32"]

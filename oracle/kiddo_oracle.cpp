@@ -1,0 +1,929 @@
+// kiddo_oracle.cpp — CPU restatement of Kiddo's ImmutableKdTree builder and scalar query
+// path (crate kiddo 6.0.0-alpha.3, default features: no `simd`, no FMA).
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the GPU engine in kiddo_amd/:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
+// product never links, imports or calls it.
+//
+// What it restates (file:line relative to the kiddo crate root):
+//   * builder          src/kd_tree/construction.rs:413-539 (new_from_source_with),
+//                      :641-691 (write_leaf_from_sort_index, soft budgets), :868-968
+//                      (populate_recursive_soft), :1089-1175 (update_pivot)
+//   * stem layout      src/stem_strategy/eytzinger/mod.rs:82-174 (root 1, child 2i+r,
+//                      dim cycles per level, leaf_idx = idx & !(1<<level))
+//   * leaf arena       src/leaf_strategy/vec_of_arenas.rs:292-323 (8-byte aligned leaf
+//                      start, tiles 32/8/4/2/1), src/leaf_view/mod.rs:18-41,431-505
+//   * traversal        src/kd_tree/orchestrator/mod.rs:388-603 (arithmetic DFS with a
+//                      restore stack and a far stack), :133-151 + traits/stem_strategy.rs
+//                      :182-200 (straight descent for approx)
+//   * metric           src/dist/squared_euclidean/mod.rs:36-39 (dist1 = d*d),
+//                      src/dist/distance_metric_core.rs:38-40,89-98 (acc += c from 0;
+//                      rd' = (rd - old^2) + new^2), src/traits/axis.rs:83-137,283-344
+//   * families         query/nearest_one.rs:111-143,321-352; nearest_n.rs:21-38;
+//                      nearest_n_within.rs:127-179,280-321,441-479; within.rs:18-41;
+//                      within_unsorted.rs:133-156,267-286; best_n_within.rs:163-196,
+//                      290-306; approx_nearest_one.rs:49-71
+//   * collectors       src/results/result_collection.rs:147-215 (ThresholdVec),
+//                      :463-561 (BinaryHeap of BestQueryResultItem), :1269-1317 (Vec)
+//   * leaf kernels     src/leaf_view_chunked/{nearest_one,nearest_n_within,
+//                      best_n_within}/fallback.rs
+//
+// Two collector modes:
+//   canonical (mode 0): results ordered by (distance, then lower item); equal-distance
+//                       subtrees are never pruned, so the whole boundary tie group is seen.
+//   faithful  (mode 1): Kiddo's own rules (distance-only ordering, strict '<' for
+//                       nearest_one and for a full k-NN collector, prune-on-equal for
+//                       nearest_one). Tie choices then depend on within-leaf order, which
+//                       Kiddo takes from Rust std's select_nth_unstable_by and which no
+//                       restatement can reproduce; this mode exists for divergence counts.
+//
+// Output type O: A for f32/f64 trees; f64 for u16/u32 trees (every value is an exact
+// integer below 2^53 there, so f64 results equal exact integer arithmetic).
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off: every product and sum is rounded
+// separately, matching Kiddo's scalar path).
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+namespace {
+
+enum { DT_F32 = 0, DT_F64 = 1, DT_U16 = 2, DT_U32 = 3 };
+constexpr int kMaxLeafLen = 1024;  // src/leaf_view/mod.rs:18, asserted construction.rs:526
+
+template <class A> struct Ax;
+template <> struct Ax<float> {
+  using O = float;
+  static float maxv() { return std::numeric_limits<float>::infinity(); }
+};
+template <> struct Ax<double> {
+  using O = double;
+  static double maxv() { return std::numeric_limits<double>::infinity(); }
+};
+template <> struct Ax<uint16_t> {
+  using O = double;
+  static uint16_t maxv() { return 0xFFFFu; }
+};
+template <> struct Ax<uint32_t> {
+  using O = double;
+  static uint32_t maxv() { return 0xFFFFFFFFu; }
+};
+
+template <class O> O out_max() { return std::numeric_limits<O>::infinity(); }
+
+// Kiddo's pop-time rule: prune if rd > bound, or rd == bound when prune-on-equal is set
+// (Axis::cmp treats NaN as Equal, axis.rs:108-120).
+template <class O> inline bool kiddo_prune(O rd, O bound, bool prune_on_equal) {
+  bool greater = rd > bound;
+  bool equal = !(rd < bound) && !greater;
+  return greater || (prune_on_equal && equal);
+}
+
+struct Layout {
+  int dtype = 0, K = 0, B = 0;
+  uint64_t n = 0;
+  int max_stem_level = -1;
+  std::vector<uint8_t> stems;      // raw A values, Eytzinger order
+  uint64_t stems_len = 0;          // number of A values
+  std::vector<uint64_t> extents;   // (byte offset, len) per leaf
+  std::vector<uint8_t> arena;      // VecOfArenas leaf_bytes
+  uint64_t max_leaf_len = 0;
+};
+
+// ---------------------------------------------------------------- builder ---------
+
+int tile_len(uint64_t remaining) {
+  // src/leaf_view/mod.rs:22-30
+  static const int widths[5] = {32, 8, 4, 2, 1};
+  for (int w : widths)
+    if (remaining >= (uint64_t)w) return w;
+  return 1;
+}
+
+template <class A, int K>
+void append_leaf(Layout& L, const A* pts, const uint32_t* items, const uint64_t* idx,
+                 uint64_t len) {
+  // src/leaf_strategy/vec_of_arenas.rs:292-323: start aligned to max(align A, align T, 8)
+  uint64_t start = (L.arena.size() + 7) & ~uint64_t(7);
+  L.arena.resize(start, 0);
+  L.extents.push_back(start);
+  L.extents.push_back(len);
+  uint64_t base = 0;
+  while (base < len) {
+    int tl = tile_len(len - base);
+    for (int d = 0; d < K; ++d)
+      for (int i = 0; i < tl; ++i) {
+        A v = pts[idx[base + i] * K + d];
+        const uint8_t* b = reinterpret_cast<const uint8_t*>(&v);
+        L.arena.insert(L.arena.end(), b, b + sizeof(A));
+      }
+    for (int i = 0; i < tl; ++i) {
+      uint32_t it = items ? items[idx[base + i]] : (uint32_t)idx[base + i];
+      const uint8_t* b = reinterpret_cast<const uint8_t*>(&it);
+      L.arena.insert(L.arena.end(), b, b + 4);
+    }
+    base += tl;
+  }
+  L.max_leaf_len = std::max<uint64_t>(L.max_leaf_len, len);
+}
+
+template <class A, int K> struct Builder {
+  const A* pts;
+  const uint32_t* items;
+  Layout& L;
+  A* stems;
+  int max_level;
+  std::vector<A> scratch;
+
+  A val(uint64_t i, int d) const { return pts[i * K + d]; }
+
+  // construction.rs:1089-1175. Kiddo selects the init_pivot-th value v, then moves the
+  // pivot left to the start of v's run of equal values; if that reaches 0 it moves right
+  // past the end of the run instead, and an all-equal slice stays unsplit (pivot = len).
+  // The restatement computes those positions from counts and partitions stably.
+  uint64_t update_pivot(uint64_t* idx, uint64_t len, int dim, uint64_t init) {
+    scratch.resize(len);
+    for (uint64_t i = 0; i < len; ++i) scratch[i] = val(idx[i], dim);
+    std::nth_element(scratch.begin(), scratch.begin() + init, scratch.end());
+    A v = scratch[init];
+    uint64_t lt = 0, le = 0;
+    for (uint64_t i = 0; i < len; ++i) {
+      A x = val(idx[i], dim);
+      lt += (x < v);
+      le += (x <= v);
+    }
+    uint64_t pivot;
+    if (lt > 0) {
+      pivot = lt;
+      std::stable_partition(idx, idx + len, [&](uint64_t i) { return val(i, dim) < v; });
+      stem_value = v;  // the run of v starts the right side
+    } else if (le < len) {
+      pivot = le;
+      std::stable_partition(idx, idx + len, [&](uint64_t i) { return val(i, dim) <= v; });
+      // Kiddo sorted the slice here, so its stem is the smallest value above v
+      stem_value = val(idx[pivot], dim);
+      for (uint64_t i = pivot; i < len; ++i) stem_value = std::min(stem_value, val(idx[i], dim));
+    } else {
+      pivot = len;
+    }
+    return pivot;
+  }
+  A stem_value{};
+
+  // construction.rs:868-968
+  void populate(uint64_t stem_idx, int level, uint64_t budget, uint64_t* idx, uint64_t len) {
+    if (budget == 0) return;
+    if (level > max_level) {
+      append_leaf<A, K>(L, pts, items, idx, len);
+      return;
+    }
+    int dim = level % K;
+    uint64_t left_budget, right_budget, pivot;
+    if (budget == 1) {
+      left_budget = 1;
+      right_budget = 0;
+      pivot = len;
+    } else {
+      // construction.rs:673-691
+      int lz = __builtin_clzll(budget - 1);
+      left_budget = uint64_t(1) << (63 - lz);
+      right_budget = budget - left_budget;
+      if (len == 0) {
+        pivot = 0;
+      } else {
+        pivot = (len * left_budget + budget - 1) / budget;
+        pivot = std::min<uint64_t>(std::max<uint64_t>(pivot, 1), len);
+      }
+      if (pivot < len) pivot = update_pivot(idx, len, dim, pivot);
+    }
+    // the stem is the smallest value of the right side (Kiddo reads it at sort_index[pivot]
+    // after select_nth / sort, construction.rs:927-933)
+    if (pivot < len) stems[stem_idx] = stem_value;
+    uint64_t split = std::min(pivot, len);
+    populate(stem_idx * 2, level + 1, left_budget, idx, split);
+    populate(stem_idx * 2 + 1, level + 1, right_budget, idx + split, len - split);
+  }
+};
+
+template <class A, int K>
+int build_layout(Layout& L, const A* pts, const uint32_t* items, uint64_t n, int B) {
+  L.K = K;
+  L.B = B;
+  L.n = n;
+  L.extents.clear();
+  L.arena.clear();
+  L.max_leaf_len = 0;
+  uint64_t leaf_node_count = (n + B - 1) / B;
+  std::vector<uint64_t> idx(n);
+  for (uint64_t i = 0; i < n; ++i) idx[i] = i;
+  if (leaf_node_count < 2) {
+    // construction.rs:540-588: one leaf, no stems
+    L.max_stem_level = -1;
+    L.stems_len = 0;
+    L.stems.clear();
+    if (n > 0) append_leaf<A, K>(L, pts, items, idx.data(), n);
+  } else {
+    int d = 0;
+    while ((uint64_t(1) << d) < leaf_node_count) ++d;  // log2(next_pow2)
+    uint64_t budget = uint64_t(1) << d;
+    uint64_t stem_count = uint64_t(1) << (d + 1);  // rightmost leaf's stem index + 1
+    uint64_t padded = (stem_count + 7) / 8 * 8;     // construction.rs:465-469
+    L.stems_len = padded;
+    L.stems.assign(padded * sizeof(A), 0);
+    A* stems = reinterpret_cast<A*>(L.stems.data());
+    for (uint64_t i = 0; i < padded; ++i) stems[i] = Ax<A>::maxv();
+    L.max_stem_level = d - 1;
+    Builder<A, K> b{pts, items, L, stems, d - 1, {}};
+    b.populate(1, 0, budget, idx.data(), n);
+  }
+  if (L.max_leaf_len > (uint64_t)kMaxLeafLen) return -2;  // construction.rs:526
+  return 0;
+}
+
+// ------------------------------------------------------------- query engine ---------
+
+struct Stats {
+  uint64_t leaves = 0, points = 0, stems = 0;
+};
+
+template <class A, int K> struct View {
+  using O = typename Ax<A>::O;
+  const A* stems;
+  const uint64_t* ext;
+  uint64_t leaf_count;
+  const uint8_t* arena;
+  uint64_t n;
+  int max_stem_level;
+
+  // src/leaf_view/mod.rs:431-446 tile walk; calls f(O dist, uint32_t item) per entry
+  // with dist computed as ((0 + dist1(x)) + dist1(y)) + ... in O, no FMA.
+  template <class F> void scan_leaf(uint64_t leaf, const O* qw, F&& f) const {
+    uint64_t off = ext[2 * leaf], len = ext[2 * leaf + 1];
+    const uint8_t* p = arena + off;
+    uint64_t base = 0;
+    while (base < len) {
+      int tl = tile_len(len - base);
+      for (int i = 0; i < tl; ++i) {
+        O acc = O(0);
+        for (int d = 0; d < K; ++d) {
+          A c;
+          std::memcpy(&c, p + (size_t(d) * tl + i) * sizeof(A), sizeof(A));
+          O diff = O(c) - qw[d];
+          acc += diff * diff;
+        }
+        uint32_t it;
+        std::memcpy(&it, p + size_t(K) * tl * sizeof(A) + size_t(i) * 4, 4);
+        f(acc, it);
+      }
+      p += size_t(K) * tl * sizeof(A) + size_t(tl) * 4;
+      base += tl;
+    }
+  }
+
+  // orchestrator/mod.rs:133-151 + traits/stem_strategy.rs:182-200
+  uint64_t straight_leaf(const A* q, Stats& st) const {
+    uint64_t idx = 1;
+    int level = 0, dim = 0;
+    while (level <= max_stem_level) {
+      A pivot = stems[idx];
+      ++st.stems;
+      bool right = q[dim] >= pivot;
+      idx = idx * 2 + (right ? 1 : 0);
+      ++level;
+      dim = (dim + 1 == K) ? 0 : dim + 1;
+    }
+    return idx & ~(uint64_t(1) << level);
+  }
+
+  // orchestrator/mod.rs:388-603. Ctx provides max_dist() (descent check), prune(rd)
+  // (re-check of a far frame when it is popped), unbounded() and leaf(leaf_idx, qw, st).
+  template <class Ctx> void dfs(const A* q, Ctx& ctx, Stats& st) const {
+    if (n == 0) return;
+    O qw[K];
+    for (int d = 0; d < K; ++d) qw[d] = O(q[d]);
+    O off[K];
+    for (int d = 0; d < K; ++d) off[d] = O(0);
+    struct Restore { O old_off; bool has_far; };
+    struct Far { uint64_t idx; int level; int dim; O far_off; O rd; };
+    // fixed-capacity stacks, like Kiddo's inline continuation stacks (query_stack.rs:5-7)
+    struct RStack {
+      Restore v[72]; int n = 0;
+      void push_back(Restore r) { v[n++] = r; }
+      Restore back() const { return v[n - 1]; }
+      void pop_back() { --n; }
+      bool empty() const { return n == 0; }
+    } rs;
+    struct FStack {
+      Far v[72]; int n = 0;
+      void push_back(Far f) { v[n++] = f; }
+      Far back() const { return v[n - 1]; }
+      void pop_back() { --n; }
+    } fs;
+    uint64_t idx = 1;
+    int level = 0, sdim = 0;
+    O rd = O(0);
+
+    auto descend = [&](bool check_bound) -> uint64_t {
+      while (level <= max_stem_level) {
+        int dim = sdim;
+        A pivot = stems[idx];
+        ++st.stems;
+        int ndim = (dim + 1 == K) ? 0 : dim + 1;
+        if (pivot < Ax<A>::maxv()) {
+          bool right = q[dim] >= pivot;
+          uint64_t near_idx = idx * 2 + (right ? 1 : 0);
+          uint64_t far_idx = idx * 2 + (right ? 0 : 1);
+          O pw = O(pivot);
+          O new_off = qw[dim] >= pw ? qw[dim] - pw : pw - qw[dim];  // |a-b| (axis.rs:124)
+          O old_off = off[dim];
+          O rd_far = (rd - old_off * old_off) + new_off * new_off;
+          bool cand = check_bound ? !(rd_far > ctx.max_dist()) : true;
+          if (cand) {
+            rs.push_back({old_off, true});
+            fs.push_back({far_idx, level + 1, ndim, new_off, rd_far});
+          } else {
+            rs.push_back({old_off, false});
+          }
+          idx = near_idx;
+        } else {
+          rs.push_back({off[dim], false});
+          idx = idx * 2;
+        }
+        ++level;
+        sdim = ndim;
+      }
+      return idx & ~(uint64_t(1) << level);
+    };
+
+    uint64_t leaf = descend(!ctx.unbounded());
+    ++st.leaves;
+    ctx.leaf(leaf, qw, st);
+    int dim = sdim;
+    for (;;) {
+      bool entered = false;
+      while (!rs.empty()) {
+        Restore fr = rs.back();
+        rs.pop_back();
+        int restore_dim = dim == 0 ? K - 1 : dim - 1;
+        off[restore_dim] = fr.old_off;
+        dim = restore_dim;
+        if (!fr.has_far) continue;
+        Far far = fs.back();
+        fs.pop_back();
+        if (ctx.prune(far.rd)) continue;  // orchestrator/mod.rs:472-483
+        rs.push_back({fr.old_off, false});
+        off[restore_dim] = far.far_off;
+        idx = far.idx;
+        level = far.level;
+        sdim = far.dim;
+        rd = far.rd;
+        leaf = descend(true);
+        ++st.leaves;
+        ctx.leaf(leaf, qw, st);
+        dim = sdim;
+        entered = true;
+        break;
+      }
+      if (!entered) break;
+    }
+  }
+};
+
+// ---- collectors / contexts
+
+template <class O> struct Res {
+  O d;
+  uint32_t it;
+};
+template <class O> inline bool canon_less(const Res<O>& a, const Res<O>& b) {
+  return a.d < b.d || (a.d == b.d && a.it < b.it);
+}
+
+template <class A, int K> struct NearestOneCtx {
+  using O = typename Ax<A>::O;
+  const View<A, K>& t;
+  bool canonical;
+  O best = out_max<O>();
+  uint32_t item = 0;  // T::default()
+  O max_dist() const { return best; }
+  bool unbounded() const { return true; }
+  // nearest_one.rs:347-351 prunes on equal; canonical keeps equal-distance subtrees
+  bool prune(O rd) const { return kiddo_prune(rd, best, !canonical); }
+  void leaf(uint64_t l, const O* qw, Stats& st) {
+    t.scan_leaf(l, qw, [&](O d, uint32_t it) {
+      ++st.points;
+      if (canonical) {
+        if (d < best || (d == best && it < item)) { best = d; item = it; }
+      } else if (d < best) {  // nearest_one/fallback.rs:98
+        best = d;
+        item = it;
+      }
+    });
+  }
+};
+
+template <class A, int K> struct NearestNCtx {
+  using O = typename Ax<A>::O;
+  const View<A, K>& t;
+  bool canonical;
+  uint64_t k;
+  O radius;
+  bool exclusive;
+  std::vector<Res<O>> v;  // faithful: ThresholdVec; canonical: sorted by (d, item)
+  bool full() const { return v.size() >= k; }
+  O max_dist() const {  // nearest_n_within.rs:461-468
+    if (full() && v.back().d < radius) return v.back().d;
+    return radius;
+  }
+  bool unbounded() const { return radius == out_max<O>(); }
+  bool prune(O rd) const {
+    if (!canonical) return kiddo_prune(rd, max_dist(), exclusive);
+    // canonical: an equal-distance subtree may hold a lower item, so only the exclusive
+    // radius prunes on equality
+    return rd > max_dist() || (exclusive && !(rd < radius));
+  }
+  void add_faithful(Res<O> e) {  // result_collection.rs:171-189
+    if (v.size() < k) {
+      v.push_back(e);
+      if (v.size() == k)
+        std::stable_sort(v.begin(), v.end(), [](const Res<O>& a, const Res<O>& b) { return a.d < b.d; });
+    } else if (e.d < v[k - 1].d) {
+      size_t i = k - 1;
+      while (i > 0 && v[i - 1].d > e.d) { v[i] = v[i - 1]; --i; }
+      v[i] = e;
+    }
+  }
+  void add_canonical(Res<O> e) {
+    if (v.size() < k) {
+      v.insert(std::upper_bound(v.begin(), v.end(), e, canon_less<O>), e);
+    } else if (canon_less(e, v[k - 1])) {
+      v.pop_back();
+      v.insert(std::upper_bound(v.begin(), v.end(), e, canon_less<O>), e);
+    }
+  }
+  void leaf(uint64_t l, const O* qw, Stats& st) {
+    O leaf_max = max_dist();  // nearest_n_within.rs:307-315: bound taken at leaf start
+    t.scan_leaf(l, qw, [&](O d, uint32_t it) {
+      ++st.points;
+      if (canonical) {
+        bool ok = exclusive ? d < radius : d <= radius;
+        if (ok) add_canonical({d, it});
+      } else {
+        bool ok = exclusive ? d < leaf_max : d <= leaf_max;
+        if (ok) add_faithful({d, it});
+      }
+    });
+  }
+};
+
+template <class A, int K> struct WithinCtx {
+  using O = typename Ax<A>::O;
+  const View<A, K>& t;
+  O radius;
+  bool exclusive;
+  std::vector<Res<O>>* out;  // nullptr: count only
+  uint64_t count = 0;
+  O max_dist() const { return radius; }
+  bool unbounded() const { return radius == out_max<O>(); }
+  bool prune(O rd) const { return kiddo_prune(rd, radius, exclusive); }
+  void leaf(uint64_t l, const O* qw, Stats& st) {
+    t.scan_leaf(l, qw, [&](O d, uint32_t it) {
+      ++st.points;
+      bool ok = exclusive ? d < radius : d <= radius;
+      if (ok) {
+        ++count;
+        if (out) out->push_back({d, it});
+      }
+    });
+  }
+};
+
+template <class A, int K> struct BestNCtx {
+  using O = typename Ax<A>::O;
+  const View<A, K>& t;
+  uint64_t nmax;
+  O radius;
+  bool exclusive;
+  std::vector<Res<O>> heap;  // max-heap by item (best_query_result_item.rs:27-31)
+  O max_dist() const { return radius; }
+  bool unbounded() const { return radius == out_max<O>(); }
+  bool prune(O rd) const { return kiddo_prune(rd, radius, exclusive); }
+  static bool item_less(const Res<O>& a, const Res<O>& b) { return a.it < b.it; }
+  void leaf(uint64_t l, const O* qw, Stats& st) {
+    bool have_thr = heap.size() >= nmax;  // best_n_within.rs: threshold_item snapshot
+    uint32_t thr = have_thr ? heap.front().it : 0;
+    t.scan_leaf(l, qw, [&](O d, uint32_t it) {
+      ++st.points;
+      bool ok = exclusive ? d < radius : d <= radius;
+      if (!ok) return;
+      if (have_thr && it >= thr) return;
+      if (heap.size() < nmax) {  // result_collection.rs:481-500
+        heap.push_back({d, it});
+        std::push_heap(heap.begin(), heap.end(), item_less);
+      } else if (it < heap.front().it) {
+        std::pop_heap(heap.begin(), heap.end(), item_less);
+        heap.back() = {d, it};
+        std::push_heap(heap.begin(), heap.end(), item_less);
+      }
+    });
+  }
+};
+
+// ------------------------------------------------------------------ helpers ---------
+
+template <class F> void parallel_for(uint64_t m, int threads, F&& f) {
+  if (threads <= 1 || m < 256) {
+    for (uint64_t i = 0; i < m; ++i) f(i, 0);
+    return;
+  }
+  std::atomic<uint64_t> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < threads; ++t)
+    th.emplace_back([&, t] {
+      for (;;) {
+        uint64_t s = next.fetch_add(1024);
+        if (s >= m) break;
+        uint64_t e = std::min<uint64_t>(m, s + 1024);
+        for (uint64_t i = s; i < e; ++i) f(i, t);
+      }
+    });
+  for (auto& x : th) x.join();
+}
+
+struct StatsAcc {
+  std::vector<Stats> per;
+  explicit StatsAcc(int threads) : per(std::max(threads, 1)) {}
+  void total(uint64_t* out) const {
+    if (!out) return;
+    Stats s;
+    for (auto& p : per) { s.leaves += p.leaves; s.points += p.points; s.stems += p.stems; }
+    out[0] = s.leaves;
+    out[1] = s.points;
+    out[2] = s.stems;
+  }
+};
+
+}  // namespace
+
+// ==================================================================== C ABI =========
+
+struct or_tree {
+  Layout L;
+};
+
+extern "C" {
+
+#define DISPATCH(dtype, K, ...)                                              \
+  switch ((dtype) * 8 + (K)) {                                                \
+    case DT_F32 * 8 + 2: { using A = float; constexpr int KK = 2; __VA_ARGS__; }     \
+    case DT_F32 * 8 + 3: { using A = float; constexpr int KK = 3; __VA_ARGS__; }     \
+    case DT_F32 * 8 + 4: { using A = float; constexpr int KK = 4; __VA_ARGS__; }     \
+    case DT_F64 * 8 + 2: { using A = double; constexpr int KK = 2; __VA_ARGS__; }    \
+    case DT_F64 * 8 + 3: { using A = double; constexpr int KK = 3; __VA_ARGS__; }    \
+    case DT_F64 * 8 + 4: { using A = double; constexpr int KK = 4; __VA_ARGS__; }    \
+    case DT_U16 * 8 + 2: { using A = uint16_t; constexpr int KK = 2; __VA_ARGS__; }  \
+    case DT_U16 * 8 + 3: { using A = uint16_t; constexpr int KK = 3; __VA_ARGS__; }  \
+    case DT_U16 * 8 + 4: { using A = uint16_t; constexpr int KK = 4; __VA_ARGS__; }  \
+    case DT_U32 * 8 + 2: { using A = uint32_t; constexpr int KK = 2; __VA_ARGS__; }  \
+    case DT_U32 * 8 + 3: { using A = uint32_t; constexpr int KK = 3; __VA_ARGS__; }  \
+    case DT_U32 * 8 + 4: { using A = uint32_t; constexpr int KK = 4; __VA_ARGS__; }  \
+    default: return -1;                                                       \
+  }
+
+// Builds a tree from n points (row-major n x K array of the dtype) exactly as
+// ImmutableKdTree::new_from_slice (items = source index) or new_from_entries (items given).
+int or_build(int dtype, int K, int B, uint64_t n, const void* pts, const uint32_t* items,
+             or_tree** out) {
+  if (B < 1 || !out) return -1;
+  auto* t = new or_tree;
+  t->L.dtype = dtype;
+  int rc = 0;
+  DISPATCH(dtype, K, rc = (build_layout<A, KK>(t->L, static_cast<const A*>(pts), items, n, B)); break);
+  if (rc != 0) {
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return 0;
+}
+
+// Wraps an existing Kiddo-layout tree (for example one made by another builder).
+int or_from_layout(int dtype, int K, int B, uint64_t n, int max_stem_level, const void* stems,
+                   uint64_t stems_len, const uint64_t* extents, uint64_t leaf_count,
+                   const uint8_t* arena, uint64_t arena_len, or_tree** out) {
+  size_t asz = dtype == DT_F32 ? 4 : dtype == DT_F64 ? 8 : dtype == DT_U16 ? 2 : dtype == DT_U32 ? 4 : 0;
+  if (!asz || K < 2 || K > 4 || !out) return -1;
+  auto* t = new or_tree;
+  Layout& L = t->L;
+  L.dtype = dtype; L.K = K; L.B = B; L.n = n; L.max_stem_level = max_stem_level;
+  L.stems_len = stems_len;
+  L.stems.assign((const uint8_t*)stems, (const uint8_t*)stems + stems_len * asz);
+  L.extents.assign(extents, extents + 2 * leaf_count);
+  L.arena.assign(arena, arena + arena_len);
+  for (uint64_t i = 0; i < leaf_count; ++i) L.max_leaf_len = std::max(L.max_leaf_len, extents[2 * i + 1]);
+  *out = t;
+  return 0;
+}
+
+void or_free(or_tree* t) { delete t; }
+
+// update_pivot on its own (construction.rs:1089-1175), for its unit test
+// (construction.rs:1187-1213). Returns the pivot; sort_index is reordered in place.
+int64_t or_update_pivot(int dtype, int K, const void* pts, uint64_t n, uint64_t* sort_index,
+                        int dim, uint64_t init) {
+  Layout L;
+  DISPATCH(dtype, K, {
+    Builder<A, KK> b{static_cast<const A*>(pts), nullptr, L, nullptr, 0, {}};
+    return (int64_t)b.update_pivot(sort_index, n, dim, init);
+  });
+}
+
+// Layout accessors (pointers stay valid until or_free).
+int or_info(const or_tree* t, int64_t* out6) {
+  // n, max_stem_level, stems_len, leaf_count, arena_len, max_leaf_len
+  out6[0] = (int64_t)t->L.n;
+  out6[1] = t->L.max_stem_level;
+  out6[2] = (int64_t)t->L.stems_len;
+  out6[3] = (int64_t)(t->L.extents.size() / 2);
+  out6[4] = (int64_t)t->L.arena.size();
+  out6[5] = (int64_t)t->L.max_leaf_len;
+  return 0;
+}
+const void* or_stems(const or_tree* t) { return t->L.stems.data(); }
+const uint64_t* or_extents(const or_tree* t) { return t->L.extents.data(); }
+const uint8_t* or_arena(const or_tree* t) { return t->L.arena.data(); }
+
+}  // extern "C"
+
+namespace {
+template <class A, int K> View<A, K> view_of(const or_tree* t) {
+  const Layout& L = t->L;
+  return View<A, K>{reinterpret_cast<const A*>(L.stems.data()), L.extents.data(),
+                    L.extents.size() / 2, L.arena.data(), L.n, L.max_stem_level};
+}
+}  // namespace
+
+extern "C" {
+
+// mode: 0 canonical, 1 faithful. dists: O per query. stats3: leaves, points, stems.
+int or_nearest_one(const or_tree* t, const void* q, uint64_t m, int mode, uint32_t* items,
+                   void* dists, int threads, uint64_t* stats3) {
+  StatsAcc acc(threads);
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      NearestOneCtx<A, KK> c{v, mode == 0};
+      v.dfs(qs + i * KK, c, acc.per[th]);
+      items[i] = c.item;
+      od[i] = c.best;
+    });
+    acc.total(stats3);
+    return 0;
+  });
+}
+
+// approx_nearest_one.rs:49-71 (requires Output == A, so only float trees).
+int or_approx_nearest_one(const or_tree* t, const void* q, uint64_t m, int mode, uint32_t* items,
+                          void* dists, int threads) {
+  if (t->L.dtype != DT_F32 && t->L.dtype != DT_F64) return -3;
+  StatsAcc acc(threads);
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      O qw[KK];
+      for (int d = 0; d < KK; ++d) qw[d] = O(qs[i * KK + d]);
+      O best = out_max<O>();
+      uint32_t item = 0;
+      if (v.n > 0) {
+        uint64_t leaf = v.straight_leaf(qs + i * KK, acc.per[th]);
+        v.scan_leaf(leaf, qw, [&](O d, uint32_t it) {
+          if (d < best || (mode == 0 && d == best && it < item)) { best = d; item = it; }
+        });
+      }
+      items[i] = item;
+      od[i] = best;
+    });
+    return 0;
+  });
+}
+
+// nearest_n (radius = +inf) and nearest_n().within(radius). Results row-major m x k;
+// rows shorter than k are padded with item 0xFFFFFFFF and distance +inf.
+int or_nearest_n(const or_tree* t, const void* q, uint64_t m, uint64_t k, double radius,
+                 int exclusive, int sorted, int mode, uint32_t* items, void* dists,
+                 uint32_t* counts, int threads, uint64_t* stats3) {
+  if (k == 0) return -1;
+  StatsAcc acc(threads);
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      NearestNCtx<A, KK> c{v, mode == 0, k, O(radius), exclusive != 0, {}};
+      c.v.reserve(k + 1);
+      v.dfs(qs + i * KK, c, acc.per[th]);
+      auto& r = c.v;
+      if (mode != 0 && sorted && r.size() < k)  // into_sorted_vec: sort when not full
+        std::stable_sort(r.begin(), r.end(), [](const Res<O>& a, const Res<O>& b) { return a.d < b.d; });
+      counts[i] = (uint32_t)r.size();
+      for (uint64_t j = 0; j < k; ++j) {
+        items[i * k + j] = j < r.size() ? r[j].it : 0xFFFFFFFFu;
+        od[i * k + j] = j < r.size() ? r[j].d : out_max<O>();
+      }
+    });
+    acc.total(stats3);
+    return 0;
+  });
+}
+
+int or_within_count(const or_tree* t, const void* q, uint64_t m, double radius, int exclusive,
+                    uint64_t* counts, int threads, uint64_t* stats3) {
+  StatsAcc acc(threads);
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      WithinCtx<A, KK> c{v, O(radius), exclusive != 0, nullptr};
+      v.dfs(qs + i * KK, c, acc.per[th]);
+      counts[i] = c.count;
+    });
+    acc.total(stats3);
+    return 0;
+  });
+}
+
+// offsets: m+1 exclusive prefix sums of the counts. sorted: 0 visit order (unsorted),
+// 1 sorted (canonical: by (dist, item); faithful: stable by distance, within.rs:18-41).
+int or_within_fill(const or_tree* t, const void* q, uint64_t m, double radius, int exclusive,
+                   int sorted, int mode, const uint64_t* offsets, uint32_t* items, void* dists,
+                   int threads) {
+  StatsAcc acc(threads);
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      std::vector<Res<O>> r;
+      WithinCtx<A, KK> c{v, O(radius), exclusive != 0, &r};
+      v.dfs(qs + i * KK, c, acc.per[th]);
+      if (sorted) {
+        if (mode == 0)
+          std::sort(r.begin(), r.end(), canon_less<O>);
+        else
+          std::stable_sort(r.begin(), r.end(), [](const Res<O>& a, const Res<O>& b) { return a.d < b.d; });
+      }
+      uint64_t o = offsets[i];
+      for (size_t j = 0; j < r.size() && o + j < offsets[i + 1]; ++j) {
+        items[o + j] = r[j].it;
+        od[o + j] = r[j].d;
+      }
+    });
+    return 0;
+  });
+}
+
+// best_n_within: rows sorted by item ascending (Kiddo returns the BinaryHeap; its
+// into_sorted_vec order is ascending item, best_n_within.rs tests).
+int or_best_n_within(const or_tree* t, const void* q, uint64_t m, double radius, uint64_t nmax,
+                     int exclusive, uint32_t* items, void* dists, uint32_t* counts, int threads,
+                     uint64_t* stats3) {
+  if (nmax == 0) return -1;
+  StatsAcc acc(threads);
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      BestNCtx<A, KK> c{v, nmax, O(radius), exclusive != 0, {}};
+      v.dfs(qs + i * KK, c, acc.per[th]);
+      auto& r = c.heap;
+      std::sort(r.begin(), r.end(), [](const Res<O>& a, const Res<O>& b) { return a.it < b.it; });
+      counts[i] = (uint32_t)r.size();
+      for (uint64_t j = 0; j < nmax; ++j) {
+        items[i * nmax + j] = j < r.size() ? r[j].it : 0xFFFFFFFFu;
+        od[i * nmax + j] = j < r.size() ? r[j].d : out_max<O>();
+      }
+    });
+    acc.total(stats3);
+    return 0;
+  });
+}
+
+// ------------------------------------------------------------ brute force ---------
+// Canonical answers by scanning every entry: the independent check of the traversal.
+
+int or_brute_nearest_n(const or_tree* t, const void* q, uint64_t m, uint64_t k, double radius,
+                       int exclusive, uint32_t* items, void* dists, uint32_t* counts, int threads) {
+  if (k == 0) return -1;
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int) {
+      O qw[KK];
+      for (int d = 0; d < KK; ++d) qw[d] = O(qs[i * KK + d]);
+      std::vector<Res<O>> r;
+      O rad = O(radius);
+      for (uint64_t l = 0; l < v.leaf_count; ++l)
+        v.scan_leaf(l, qw, [&](O d, uint32_t it) {
+          bool ok = exclusive ? d < rad : d <= rad;
+          if (!ok) return;
+          Res<O> e{d, it};
+          if (r.size() < k) {
+            r.insert(std::upper_bound(r.begin(), r.end(), e, canon_less<O>), e);
+          } else if (canon_less(e, r.back())) {
+            r.pop_back();
+            r.insert(std::upper_bound(r.begin(), r.end(), e, canon_less<O>), e);
+          }
+        });
+      counts[i] = (uint32_t)r.size();
+      for (uint64_t j = 0; j < k; ++j) {
+        items[i * k + j] = j < r.size() ? r[j].it : 0xFFFFFFFFu;
+        od[i * k + j] = j < r.size() ? r[j].d : out_max<O>();
+      }
+    });
+    return 0;
+  });
+}
+
+int or_brute_within(const or_tree* t, const void* q, uint64_t m, double radius, int exclusive,
+                    const uint64_t* offsets, uint64_t* counts, uint32_t* items, void* dists,
+                    int threads) {
+  // offsets == nullptr: count only. Otherwise fill sorted by (dist, item).
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int) {
+      O qw[KK];
+      for (int d = 0; d < KK; ++d) qw[d] = O(qs[i * KK + d]);
+      std::vector<Res<O>> r;
+      O rad = O(radius);
+      for (uint64_t l = 0; l < v.leaf_count; ++l)
+        v.scan_leaf(l, qw, [&](O d, uint32_t it) {
+          bool ok = exclusive ? d < rad : d <= rad;
+          if (ok) r.push_back({d, it});
+        });
+      if (counts) counts[i] = r.size();
+      if (offsets) {
+        std::sort(r.begin(), r.end(), canon_less<O>);
+        for (size_t j = 0; j < r.size(); ++j) {
+          items[offsets[i] + j] = r[j].it;
+          od[offsets[i] + j] = r[j].d;
+        }
+      }
+    });
+    return 0;
+  });
+}
+
+int or_brute_best_n_within(const or_tree* t, const void* q, uint64_t m, double radius,
+                           uint64_t nmax, int exclusive, uint32_t* items, void* dists,
+                           uint32_t* counts, int threads) {
+  if (nmax == 0) return -1;
+  DISPATCH(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int) {
+      O qw[KK];
+      for (int d = 0; d < KK; ++d) qw[d] = O(qs[i * KK + d]);
+      std::vector<Res<O>> r;
+      O rad = O(radius);
+      for (uint64_t l = 0; l < v.leaf_count; ++l)
+        v.scan_leaf(l, qw, [&](O d, uint32_t it) {
+          bool ok = exclusive ? d < rad : d <= rad;
+          if (ok) r.push_back({d, it});
+        });
+      std::sort(r.begin(), r.end(), [](const Res<O>& a, const Res<O>& b) { return a.it < b.it; });
+      if (r.size() > nmax) r.resize(nmax);
+      counts[i] = (uint32_t)r.size();
+      for (uint64_t j = 0; j < nmax; ++j) {
+        items[i * nmax + j] = j < r.size() ? r[j].it : 0xFFFFFFFFu;
+        od[i * nmax + j] = j < r.size() ? r[j].d : out_max<O>();
+      }
+    });
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,245 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the oracle on the same seeded
+inputs, bit for bit: items, result order (distance, then lower item) and distance bits.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from engines import (GpuEngine, OracleEngine, assert_same_bits, load_kats, run_kat_case,
+                     scalar_order_sqdist)
+
+pytestmark = pytest.mark.gpu
+
+KATS = load_kats()
+
+
+def _announce(*what):
+    print("case:", *what, flush=True)
+
+
+@pytest.mark.parametrize("case", KATS["cases"], ids=lambda c: c["name"])
+def test_reference_kats_on_gpu(ka, cuda, case):
+    _announce("kat", case["name"])
+    run_kat_case(GpuEngine(ka), case)
+    run_kat_case(GpuEngine(ka, flags=ka.KD_MORTON), case)
+
+
+def _points(kind, n, K, dtype, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "uniform":
+        p = rng.random((n, K))
+    elif kind == "dups":
+        base = rng.random((max(n // 4, 1), K))
+        p = base[rng.integers(0, len(base), n)]
+    elif kind == "grid":
+        p = rng.integers(0, 7, (n, K)).astype(np.float64)
+    elif kind == "mixture":
+        from kiddo_amd.datasets import gaussian_mixture
+        p = gaussian_mixture(n, 8, 0.02, K, seed=seed).astype(np.float64)
+    elif kind == "all_same":
+        p = np.full((n, K), 0.375)
+    else:
+        raise ValueError(kind)
+    if np.dtype(dtype).kind == "u":
+        if kind != "grid":
+            p = p * 60000
+        return p.astype(dtype)
+    return p.astype(dtype)
+
+
+def _queries(pts, m, seed):
+    rng = np.random.default_rng(seed + 99)
+    if pts.dtype.kind == "u":
+        q = rng.integers(0, int(pts.max()) + 2, (m, pts.shape[1])).astype(pts.dtype)
+    else:
+        lo, hi = float(pts.min()), float(pts.max())
+        span = max(hi - lo, 1.0)
+        q = (lo - 0.05 * span + rng.random((m, pts.shape[1])) * 1.1 * span).astype(pts.dtype)
+    if len(pts):
+        q[: m // 4] = pts[rng.integers(0, len(pts), m // 4)]
+    return q
+
+
+def _radius(pts):
+    if pts.dtype.kind == "u":
+        return 2.0 if pts.max() < 100 else 2.5e7
+    return 2.0 if pts.max() > 1.5 else 0.004
+
+
+CASES = [
+    ("uniform", 100_000, 3, np.float64, 32),
+    ("uniform", 100_000, 3, np.float32, 32),
+    ("dups", 60_000, 3, np.float64, 32),
+    ("grid", 40_000, 3, np.float64, 32),
+    ("grid", 30_000, 2, np.float32, 8),
+    ("mixture", 80_000, 3, np.float32, 32),
+    ("uniform", 50_000, 4, np.float64, 16),
+    ("uniform", 65_536, 4, np.uint16, 32),
+    ("grid", 20_000, 3, np.uint16, 32),
+    ("uniform", 30_000, 3, np.uint32, 32),
+    ("all_same", 800, 3, np.float64, 32),
+    ("uniform", 20, 3, np.float64, 32),
+    ("uniform", 1, 3, np.float32, 32),
+]
+IDS = [f"{c[0]}-{c[1]}-K{c[2]}-{np.dtype(c[3]).name}-B{c[4]}" for c in CASES]
+
+
+@pytest.mark.parametrize("kind,n,K,dtype,B", CASES, ids=IDS)
+def test_families_match_oracle(ka, oracle_mod, cuda, kind, n, K, dtype, B):
+    _announce("families", kind, n, K, np.dtype(dtype).name, B)
+    pts = _points(kind, n, K, dtype, seed=n + K)
+    q = _queries(pts, 4000, seed=n)
+    r = _radius(pts)
+    O = OracleEngine(oracle_mod)
+    ot = O.build(pts, B)
+    for flags in (0, ka.KD_MORTON):
+        G = GpuEngine(ka, flags=flags)
+        gt = G.build(pts, B)
+        tag = f"flags={flags}"
+        gi, gd = G.nearest_one(gt, q)
+        oi, od = O.nearest_one(ot, q)
+        assert_same_bits(gi, oi, f"nearest_one items {tag}")
+        assert_same_bits(gd, od, f"nearest_one dists {tag}")
+        for k in (1, 5, 16):
+            res_g = G.nearest_n(gt, q, k)
+            res_o = O.nearest_n(ot, q, k)
+            for a, b, nm in zip(res_g, res_o, ("items", "dists", "counts")):
+                assert_same_bits(a, b, f"nearest_n({k}) {nm} {tag}")
+        for excl in (False, True):
+            res_g = G.nearest_n(gt, q, 8, r, excl)
+            res_o = O.nearest_n(ot, q, 8, r, excl)
+            for a, b, nm in zip(res_g, res_o, ("items", "dists", "counts")):
+                assert_same_bits(a, b, f"nearest_n_within {nm} excl={excl} {tag}")
+            go, gi, gd = G.within(gt, q, r, excl, sorted=True)
+            oo, oi, od = O.within(ot, q, r, excl, sorted=True)
+            assert_same_bits(go, oo, f"within offsets excl={excl} {tag}")
+            assert_same_bits(gi, oi, f"within items excl={excl} {tag}")
+            assert_same_bits(gd, od, f"within dists excl={excl} {tag}")
+            uo, ui, ud = G.within(gt, q, r, excl, sorted=False)
+            assert_same_bits(uo, oo, "within unsorted offsets")
+            # unsorted: the same multiset per query
+            key_g = np.lexsort((ui, ud, np.repeat(np.arange(len(q)), np.diff(uo.astype(np.int64)))))
+            assert_same_bits(ui[key_g], oi, "within unsorted items (as multiset)")
+            assert_same_bits(ud[key_g], od, "within unsorted dists (as multiset)")
+            for nn in (1, 5):
+                res_g = G.best_n_within(gt, q, r, nn, excl)
+                res_o = O.best_n_within(ot, q, r, nn, excl)
+                for a, b, nm in zip(res_g, res_o, ("items", "dists", "counts")):
+                    assert_same_bits(a, b, f"best_n_within({nn}) {nm} excl={excl} {tag}")
+        if pts.dtype.kind == "f":
+            gi, gd = G.approx_nearest_one(gt, q)
+            oi, od = O.approx_nearest_one(ot, q)
+            assert_same_bits(gi, oi, f"approx items {tag}")
+            assert_same_bits(gd, od, f"approx dists {tag}")
+
+
+def test_large_k_and_k_above_n(ka, oracle_mod, cuda):
+    _announce("large k")
+    pts = _points("uniform", 3000, 3, np.float64, 1)
+    q = _queries(pts, 500, 1)
+    O, G = OracleEngine(oracle_mod), GpuEngine(ka)
+    ot, gt = O.build(pts, 32), G.build(pts, 32)
+    for k in (24, 32, 64):
+        for a, b in zip(G.nearest_n(gt, q, k), O.nearest_n(ot, q, k)):
+            assert_same_bits(a, b, f"k={k}")
+    small = _points("uniform", 7, 3, np.float64, 2)
+    ot, gt = O.build(small, 32), G.build(small, 32)
+    for a, b in zip(G.nearest_n(gt, q, 16), O.nearest_n(ot, q, 16)):
+        assert_same_bits(a, b, "k > n")
+
+
+def test_empty_tree(ka, cuda):
+    _announce("empty tree")
+    G = GpuEngine(ka)
+    t = G.build(np.zeros((0, 3)), 32)
+    q = np.array([[0.1, 0.2, 0.3], [0.5, 0.5, 0.5]])
+    it, d = G.nearest_one(t, q)
+    assert it.tolist() == [0, 0] and np.all(np.isinf(d))
+    it, d, c = G.nearest_n(t, q, 4)
+    assert c.tolist() == [0, 0] and np.all(it == 0xFFFFFFFF)
+    off, wi, wd = G.within(t, q, 1.0)
+    assert off.tolist() == [0, 0, 0]
+
+
+def test_screen_f32_equals_plain_f64(ka, oracle_mod, cuda):
+    """FP32 screening + FP64 rescore returns exactly the plain FP64 kernel's results."""
+    _announce("screen f32")
+    from kiddo_amd.datasets import uniform_with_duplicates
+    pts = uniform_with_duplicates(200_000, 0.05, seed=3)
+    q = _queries(pts, 20000, 3)
+    plain = ka.ImmutableKdTree.new_from_slice(pts, 32)
+    shadow = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
+    O = OracleEngine(oracle_mod)
+    ot = O.build(pts, 32)
+    for k in (1, 8, 16):
+        ref = O.nearest_n(ot, q, k)
+        for flags in (ka.KD_SCREEN_F32, ka.KD_SCREEN_F32 | ka.KD_MORTON):
+            r = shadow.nearest_n_raw(q, k, flags=flags)
+            got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
+            for a, b in zip(got, ref):
+                assert_same_bits(a, b, f"screen k={k} flags={flags}")
+        screened, rescored = shadow.screen_stats()
+        assert 0 < rescored < screened
+        r = plain.nearest_n_raw(q, k)
+        assert_same_bits(r.items.cpu().numpy(), ref[0])
+    # radius-limited and exclusive forms
+    for excl in (False, True):
+        ref = O.nearest_n(ot, q, 16, 4e-6, excl)
+        r = shadow.nearest_n_raw(q, 16, 4e-6, ka.KD_SCREEN_F32 | (ka.KD_EXCLUSIVE if excl else 0))
+        assert_same_bits(r.items.cpu().numpy(), ref[0], "screen within items")
+        assert_same_bits(r.distances.cpu().numpy(), ref[1], "screen within dists")
+
+
+def test_distances_match_numpy_scalar_order(ka, cuda):
+    _announce("numpy scalar order")
+    rng = np.random.default_rng(4)
+    for dtype in (np.float64, np.float32):
+        pts = rng.random((50_000, 3)).astype(dtype)
+        q = rng.random((2000, 3)).astype(dtype)
+        t = ka.ImmutableKdTree.new_from_slice(pts, 32)
+        r = t.query(q).nearest_n(16).execute()
+        it = r.items.cpu().numpy()
+        ref = scalar_order_sqdist(pts[it], q[:, None, :], dtype)
+        assert_same_bits(r.distances.cpu().numpy(), ref, np.dtype(dtype).name)
+
+
+def test_builder_surface(ka, cuda):
+    _announce("builder surface")
+    pts = np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0], [0.5, 0.0]])
+    t = ka.ImmutableKdTree.new_from_slice(pts, 32)
+    q = np.array([[0.0, 0.0]])
+    r = t.query(q).nearest_n(8).within(1.0).exclusive_boundaries().execute()
+    assert r.items.cpu().numpy()[0][: int(r.counts[0])].tolist() == [0, 3]
+    r = t.query(q).within(1.0).unsorted().execute()
+    assert sorted(r.items.cpu().numpy().tolist()) == [0, 1, 3]
+    r = t.query(q).nearest_one().approx().execute()
+    assert int(r.item[0]) == 0
+    with pytest.raises(ValueError):
+        t.query(q).within(1.0).approx()
+    with pytest.raises(ka.KiddoAmdError):
+        t.query(q).nearest_n(8).screen_f32().execute()  # no FP32 shadow on this tree
+
+
+def test_traversal_visits_match_oracle(ka, oracle_mod, cuda):
+    """The kernels visit exactly the leaves (and scan exactly the entries) that the
+    oracle's restatement of Kiddo's DFS visits for the same queries."""
+    _announce("visit counters")
+    rng = np.random.default_rng(12)
+    pts = rng.random((100_000, 3))
+    q = rng.random((5000, 3))
+    t = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
+    ot = oracle_mod.Tree.build(pts, 32)
+    for flags in (0, ka.KD_MORTON, ka.KD_SCREEN_F32 | ka.KD_MORTON):
+        t.nearest_n_raw(q, 16, flags=flags)
+        st = t.query_stats()
+        _, _, _, ost = ot.nearest_n(q, 16, stats=True)
+        assert st["leaves"] == int(ost[0]), (flags, st, ost)
+        assert st["points"] == int(ost[1]), (flags, st, ost)
+    t.nearest_one_raw(q)
+    st = t.query_stats()
+    _, _, ost = ot.nearest_one(q, stats=True)
+    assert (st["leaves"], st["points"]) == (int(ost[0]), int(ost[1]))
+    t.set_timing(True)
+    t.nearest_n_raw(q, 16)
+    assert t.last_kernel_ms() > 0
