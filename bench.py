@@ -92,11 +92,19 @@ def main():
     flags = {"plain": 0, "screened": ka.KD_SCREEN_F32 | ka.KD_MORTON}
     variants = ["plain", "screened"] if args.only == "both" else [args.only]
 
+    # plain: one thread per query, queries in input order, no LDS stems. screened: the
+    # launch shape the autotuner picks on a sample of the batch.
+    tuning = {"plain": dict(variant=1, group_size=0, lds_levels=0)}
+    if "screened" in variants:
+        tuning["screened"] = tree.autotune_nearest_n(q, k, math.inf, flags["screened"])
+        log(f"autotuned screened launch: {tuning['screened']}")
+
     def run(name, qq=q, out=None):
         tree.nearest_n_raw(qq, k, math.inf, flags[name], out=out or outs[name])
 
     results = {}
     for name in variants:
+        tree.tune(tuning[name]["variant"], tuning[name]["group_size"], tuning[name]["lds_levels"])
         for _ in range(args.warmup):
             run(name)
         torch.cuda.synchronize()
@@ -122,7 +130,7 @@ def main():
     head = "screened" if "screened" in results else "plain"
 
     # latency of fixed-size sub-batches (screened path)
-    lat = []
+    tree.tune(tuning[head]["variant"], tuning[head]["group_size"], tuning[head]["lds_levels"])
     lb = min(args.latency_batch, m)
     nb = m // lb
     sub = ka.NearestN(torch.empty((lb, k), dtype=torch.uint32, device=dev),
@@ -144,8 +152,9 @@ def main():
     if "screened" in results:
         rng = np.random.default_rng(7)
         rows = rng.integers(0, m, 200_000)
-        it = outs["screened"].items[torch.from_numpy(rows).to(dev)].cpu().numpy().astype(np.int64)
-        d64 = outs["screened"].distances[torch.from_numpy(rows).to(dev)].cpu().numpy()
+        ridx = torch.from_numpy(rows).to(dev)
+        it = outs["screened"].items.view(torch.int32)[ridx].cpu().numpy().view(np.uint32).astype(np.int64)
+        d64 = outs["screened"].distances[ridx].cpu().numpy()
         p32 = pts[it].astype(np.float32)
         q32 = qh[rows][:, None, :].astype(np.float32)
         acc = np.zeros(d64.shape, np.float32)
@@ -209,7 +218,8 @@ def main():
         "data": "synthetic (seeded: uniform + 5% exact duplicates; uniform queries)",
         "config": {"workload": "knn16-f64-3d-16M+5%dup-16Mq", "points": int(pts.shape[0]),
                    "queries": m, "k": k, "bucket": args.bucket,
-                   "variant": "FP32-screened, Morton-ordered" if head == "screened" else "plain FP64"},
+                   "variant": "FP32-screened, Morton-ordered" if head == "screened" else "plain FP64",
+                   "launch": tuning[head]},
         "plain_f64_qps": round(results["plain"]["qps"], 1) if "plain" in results else None,
         "screened_over_plain": (round(results["screened"]["qps"] / results["plain"]["qps"], 3)
                                 if len(results) == 2 else None),

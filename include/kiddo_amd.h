@@ -108,6 +108,19 @@ int kd_tree_create(const kd_tree_desc* desc, uint32_t tree_flags, kd_tree** out)
 int kd_tree_destroy(kd_tree* tree);
 int kd_tree_get_info(const kd_tree* tree, kd_tree_info* out);
 
+/* Launch tuning (no Kiddo counterpart; results never depend on it).
+ * knn_variant: 0 automatic, 1 one thread per query, 2 one group of group_size lanes per
+ * query (group_size 0 = smallest of 4/8/16/32/64 that holds k). lds_levels: top stem
+ * levels copied into LDS per block (-1 automatic, 0 none, at most 14). */
+int kd_tree_tune(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_t lds_levels);
+
+/* Autotuner: times each candidate launch shape (thread vs group per query, group size,
+ * LDS stem levels) on up to 2^18 of the given queries with the given k/radius/flags, keeps
+ * the fastest in the tree (as kd_tree_tune would) and reports it in chosen3 = {variant,
+ * group_size, lds_levels}. Synchronous. */
+int kd_autotune_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k,
+                          double radius, uint32_t flags, int32_t* chosen3, void* stream);
+
 /* --------------------------------------------------------------------- queries ---- */
 /* tree.query(q).nearest_one::<SquaredEuclidean>().execute()  (builder.rs:2008-2017;
  * query/nearest_one.rs:61-143). items[m], dists[m] (O). Empty tree: (+inf, 0). */

@@ -74,6 +74,8 @@ SIGNATURES = {
     "kd_tree_create": ([C.POINTER(TreeDesc), _u32, C.POINTER(_vp)], C.c_int),
     "kd_tree_destroy": ([_vp], C.c_int),
     "kd_tree_get_info": ([_vp, C.POINTER(TreeInfo)], C.c_int),
+    "kd_tree_tune": ([_vp, _i32, _i32, _i32], C.c_int),
+    "kd_autotune_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp], C.c_int),
     "kd_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_approx_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
@@ -95,6 +97,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise KiddoAmdError(
                 f"{LIB_PATH} is missing: build it with `python -m kiddo_amd._build`")
+        # torch carries its own HIP runtime (sonames libamdhip64.so.7 and
+        # libhsa-runtime64.so.1). Loading torch first makes the engine bind to that same
+        # runtime by soname; otherwise a second runtime would be loaded and find no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             f = getattr(L, name)
@@ -278,6 +287,19 @@ class ImmutableKdTree:
         out = np.zeros(4, np.uint64)
         _check(lib().kd_query_stats(self._h, _np_ptr(out)), "kd_query_stats")
         return dict(zip(("leaves", "points", "stems", "rescored"), (int(x) for x in out)))
+
+    def tune(self, knn_variant: int = 0, group_size: int = 0, lds_levels: int = -1):
+        """Kernel form for k-NN (0 auto, 1 thread/query, 2 group/query) and LDS stem levels."""
+        _check(lib().kd_tree_tune(self._h, knn_variant, group_size, lds_levels), "kd_tree_tune")
+
+    def autotune_nearest_n(self, q, k, radius=math.inf, flags=0, stream=None):
+        """Times the candidate launch shapes on a sample of q and keeps the fastest."""
+        q = self._queries(q)
+        chosen = np.zeros(3, np.int32)
+        _check(lib().kd_autotune_nearest_n(self._h, q.data_ptr(), q.shape[0], k, float(radius),
+                                           flags, _np_ptr(chosen), _stream_ptr(stream)),
+               "kd_autotune_nearest_n")
+        return dict(zip(("variant", "group_size", "lds_levels"), (int(x) for x in chosen)))
 
     def set_timing(self, enable: bool = True):
         _check(lib().kd_tree_set_timing(self._h, int(enable)), "kd_tree_set_timing")

@@ -28,20 +28,44 @@ HIP_FLAGS = [
 ]
 
 
+SOURCES = ["api.hip", "k_knn.hip", "k_families.hip", "k_forest.hip", "host_build.cpp"]
+OBJ_DIR = os.path.join(HERE, "_build_obj")
+
+
 def sources():
-    return [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "host_build.cpp")]
+    return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
 def build(verbose: bool = False) -> str:
+    """Compiles every translation unit in parallel, then links the shared library."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    procs = []
+    objs = []
+    for src in sources():
+        obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmd = [hipcc, *HIP_FLAGS, "-c", "-o", obj, src]
+        if src.endswith(".cpp"):
+            cmd = [hipcc, "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
+    failed = []
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append(src)
+            sys.stderr.write(out)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {failed}")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIP_FLAGS, "-shared", "-pthread", "-o", tmp, *sources()]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-pthread", "-fPIC", "-o", tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed to build libkiddo_amd.so")
+        raise RuntimeError("link of libkiddo_amd.so failed")
     os.replace(tmp, LIB)
     return LIB
 

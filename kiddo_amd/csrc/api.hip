@@ -1,0 +1,601 @@
+// api.hip — C ABI of the engine (include/kiddo_amd.h): tree upload and re-layout,
+// Morton ordering of query batches, launch tuning, timing, and the per-family entry points
+// (whose kernels live in k_knn.hip and k_families.hip).
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "tree.h"
+
+// ------------------------------------------------------------------- errors --------
+namespace kd {
+namespace {
+thread_local std::string g_last_error;
+}
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+}  // namespace kd
+
+
+namespace kd {
+namespace dev {
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+// Clears n counters (a kernel rather than hipMemsetAsync so that callers may capture it).
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
+
+}  // namespace dev
+
+uint32_t grid_for(uint64_t threads) { return uint32_t((threads + dev::kBlock - 1) / dev::kBlock); }
+
+void reset_counters(kd_tree* t, hipStream_t s) {
+  hipLaunchKernelGGL(dev::k_zero_u64, dim3(1), dim3(64), 0, s, t->counters, 4);
+}
+
+uint32_t lds_stems_for(const kd_tree* t, size_t a_size) {
+  // Default: the top levels that fit in 16 KiB, so that eight 256-thread blocks can
+  // share a CU's 160 KiB of LDS.
+  int levels = t->lds_levels;
+  if (levels < 0) {
+    levels = 0;
+    while (levels < 14 && (size_t(1) << (levels + 1)) * a_size <= 16384) ++levels;
+  }
+  const uint64_t want = levels > 0 ? (uint64_t(1) << levels) : 0;
+  return uint32_t(std::min<uint64_t>(want, t->stems_len));
+}
+
+namespace {
+
+void free_tree(kd_tree* t) {
+  if (!t) return;
+  if (t->ev_start) (void)hipEventDestroy(t->ev_start);
+  if (t->ev_stop) (void)hipEventDestroy(t->ev_stop);
+  (void)hipFree(t->stems);
+  (void)hipFree(t->leaf_start);
+  (void)hipFree(t->coords);
+  (void)hipFree(t->items);
+  (void)hipFree(t->shadow);
+  (void)hipFree(t->d_max_abs);
+  (void)hipFree(t->d_bbox);
+  (void)hipFree(t->counters);
+  delete t;
+}
+
+template <class T> int dmalloc(T** p, size_t bytes, uint64_t& total) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(KD_ERR_NO_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  total += bytes;
+  return KD_OK;
+}
+
+#define KD_TRY(x)           \
+  do {                      \
+    int rc_ = (x);          \
+    if (rc_ != KD_OK) {     \
+      free_tree(t);         \
+      return rc_;           \
+    }                       \
+  } while (0)
+
+template <class F> void host_parallel(uint64_t count, F&& f) {
+  int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (count < 65536) threads = 1;
+  std::vector<std::thread> th;
+  uint64_t chunk = (count + threads - 1) / threads;
+  for (int i = 0; i < threads; ++i) {
+    uint64_t s = i * chunk, e = std::min(count, s + chunk);
+    if (s >= e) break;
+    th.emplace_back([&f, s, e] { f(s, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+// Decodes Kiddo's leaf arena into SoA + items, validates it, and uploads everything.
+template <class A, int K> int create_typed(const kd_tree_desc* desc, uint32_t flags, kd_tree** out) {
+  const uint64_t n = desc->size, L = desc->leaf_count;
+  const int msl = desc->max_stem_level;
+  if (msl >= kMaxStemLevels) return set_error(KD_ERR_TOO_DEEP, "max_stem_level %d too deep", msl);
+  if (msl < -1) return set_error(KD_ERR_INVALID, "max_stem_level %d invalid", msl);
+  const uint64_t want_leaves = msl >= 0 ? (uint64_t(1) << (msl + 1)) : (n > 0 ? 1 : 0);
+  if (L != want_leaves && !(msl < 0 && L <= 1))
+    return set_error(KD_ERR_INVALID, "leaf_count %llu does not match max_stem_level %d",
+                     (unsigned long long)L, msl);
+  if (msl >= 0 && desc->stems_len < (uint64_t(1) << (msl + 1)))
+    return set_error(KD_ERR_INVALID, "stems_len %llu too short", (unsigned long long)desc->stems_len);
+  if (n > 0xFFFFFFFFull) return set_error(KD_ERR_INVALID, "trees above 2^32-1 entries are not supported");
+  if ((flags & KD_TREE_SHADOW_F32) && (desc->dtype != KD_F64 || K > 3))
+    return set_error(KD_ERR_UNSUPPORTED, "the FP32 shadow is provided for f64 trees with dims <= 3");
+  const size_t entry_bytes = K * sizeof(A) + 4;
+  std::vector<uint32_t> leaf_start(L + 1, 0);
+  uint64_t total = 0, max_len = 0;
+  for (uint64_t l = 0; l < L; ++l) {
+    const uint64_t off = desc->leaf_extents[2 * l], len = desc->leaf_extents[2 * l + 1];
+    if (len > uint64_t(kMaxLeafLen))
+      return set_error(KD_ERR_LEAF_OVERFLOW, "leaf %llu holds %llu entries (> 1024)",
+                       (unsigned long long)l, (unsigned long long)len);
+    if (off + len * entry_bytes > desc->leaf_bytes_len)
+      return set_error(KD_ERR_INVALID, "leaf %llu extends past the arena", (unsigned long long)l);
+    leaf_start[l] = uint32_t(total);
+    total += len;
+    max_len = std::max(max_len, len);
+  }
+  leaf_start[L] = uint32_t(total);
+  if (total != n)
+    return set_error(KD_ERR_INVALID, "leaf lengths sum to %llu, size is %llu",
+                     (unsigned long long)total, (unsigned long long)n);
+
+  const uint64_t plane = (n + 63) / 64 * 64;
+  std::vector<A> coords(plane * K);
+  std::vector<uint32_t> items(n);
+  std::vector<float4> shadow((flags & KD_TREE_SHADOW_F32) ? n : 0);
+  // per-thread partial bounding boxes
+  std::mutex mu;
+  double lo[4] = {INFINITY, INFINITY, INFINITY, INFINITY}, hi[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  double mabs[4] = {0, 0, 0, 0};
+  bool bad = false;
+  host_parallel(L, [&](uint64_t s, uint64_t e) {
+    double tlo[4] = {INFINITY, INFINITY, INFINITY, INFINITY}, thi[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    double tabs[4] = {0, 0, 0, 0};
+    bool tbad = false;
+    for (uint64_t l = s; l < e; ++l) {
+      const uint8_t* p = desc->leaf_bytes + desc->leaf_extents[2 * l];
+      const uint64_t len = desc->leaf_extents[2 * l + 1];
+      uint64_t dst = leaf_start[l], base = 0;
+      while (base < len) {
+        const uint64_t rem = len - base;
+        const int tl = rem >= 32 ? 32 : rem >= 8 ? 8 : rem >= 4 ? 4 : rem >= 2 ? 2 : 1;
+        for (int d = 0; d < K; ++d)
+          for (int i = 0; i < tl; ++i) {
+            A v;
+            std::memcpy(&v, p + (size_t(d) * tl + i) * sizeof(A), sizeof(A));
+            coords[d * plane + dst + i] = v;
+            const double x = double(v);
+            if (std::isnan(x) || v == AxisHost<A>::max_value()) tbad = true;
+            tlo[d] = std::min(tlo[d], x);
+            thi[d] = std::max(thi[d], x);
+            tabs[d] = std::max(tabs[d], std::fabs(x));
+          }
+        for (int i = 0; i < tl; ++i) std::memcpy(&items[dst + i], p + size_t(K) * tl * sizeof(A) + 4 * i, 4);
+        p += size_t(K) * tl * sizeof(A) + 4 * size_t(tl);
+        dst += tl;
+        base += tl;
+      }
+    }
+    std::lock_guard<std::mutex> g(mu);
+    for (int d = 0; d < 4; ++d) {
+      lo[d] = std::min(lo[d], tlo[d]);
+      hi[d] = std::max(hi[d], thi[d]);
+      mabs[d] = std::max(mabs[d], tabs[d]);
+    }
+    bad |= tbad;
+  });
+  if (bad) return set_error(KD_ERR_INVALID, "tree holds a NaN coordinate or the reserved maximum value");
+  if (!shadow.empty()) {
+    for (int d = 0; d < K; ++d)
+      if (!(mabs[d] <= 0x1p60))
+        return set_error(KD_ERR_UNSUPPORTED, "FP32 shadow needs |coordinates| <= 2^60");
+    host_parallel(n, [&](uint64_t s, uint64_t e) {
+      for (uint64_t i = s; i < e; ++i) {
+        float4 f;
+        f.x = float(coords[0 * plane + i]);
+        f.y = float(coords[1 * plane + i]);
+        f.z = K > 2 ? float(coords[2 * plane + i]) : 0.0f;
+        std::memcpy(&f.w, &items[i], 4);
+        shadow[i] = f;
+      }
+    });
+  }
+
+  kd_tree* t = new kd_tree;
+  t->dtype = desc->dtype;
+  t->dims = K;
+  t->bucket = desc->bucket;
+  t->max_stem_level = msl;
+  t->size = n;
+  t->leaf_count = L;
+  t->max_leaf_len = max_len;
+  t->plane = plane;
+  t->stems_len = desc->stems_len;
+  (void)hipGetDevice(&t->device);
+  for (int d = 0; d < K; ++d) {
+    t->bbox_min[d] = n ? lo[d] : 0.0;
+    t->bbox_max[d] = n ? hi[d] : 0.0;
+  }
+  uint64_t& tot = t->device_bytes;
+  KD_TRY(dmalloc(&t->stems, std::max<uint64_t>(desc->stems_len, 1) * sizeof(A), tot));
+  KD_TRY(dmalloc(&t->leaf_start, (L + 1) * sizeof(uint32_t), tot));
+  KD_TRY(dmalloc(&t->coords, coords.size() * sizeof(A), tot));
+  KD_TRY(dmalloc(&t->items, std::max<uint64_t>(n, 1) * sizeof(uint32_t), tot));
+  KD_TRY(dmalloc(&t->d_max_abs, 4 * sizeof(double), tot));
+  KD_TRY(dmalloc(&t->d_bbox, 8 * sizeof(double), tot));
+  KD_TRY(dmalloc(&t->counters, 4 * sizeof(unsigned long long), tot));
+  if (!shadow.empty()) KD_TRY(dmalloc(&t->shadow, shadow.size() * sizeof(float4), tot));
+  double bb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int d = 0; d < K; ++d) {
+    bb[d] = t->bbox_min[d];
+    const double ext = t->bbox_max[d] - t->bbox_min[d];
+    bb[4 + d] = ext > 0 ? double((1u << (32 / K)) - 1u) / ext : 0.0;
+  }
+  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes == 0) return KD_OK;
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_error(KD_ERR_DEVICE, "upload failed: %s", hipGetErrorString(e));
+    return KD_OK;
+  };
+  if (desc->stems_len) KD_TRY(up(t->stems, desc->stems, desc->stems_len * sizeof(A)));
+  KD_TRY(up(t->leaf_start, leaf_start.data(), leaf_start.size() * sizeof(uint32_t)));
+  KD_TRY(up(t->coords, coords.data(), coords.size() * sizeof(A)));
+  KD_TRY(up(t->items, items.data(), items.size() * sizeof(uint32_t)));
+  KD_TRY(up(t->d_max_abs, mabs, 4 * sizeof(double)));
+  KD_TRY(up(t->d_bbox, bb, 8 * sizeof(double)));
+  if (!shadow.empty()) KD_TRY(up(t->shadow, shadow.data(), shadow.size() * sizeof(float4)));
+  if (hipMemset(t->counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    free_tree(t);
+    return set_error(KD_ERR_DEVICE, "hipMemset failed");
+  }
+  *out = t;
+  return KD_OK;
+}
+
+// Optional Morton ordering of a batch: returns a device permutation (or null).
+struct Order {
+  uint32_t* perm = nullptr;
+  void* mem = nullptr;
+  hipStream_t s = nullptr;
+  ~Order() {
+    if (mem) (void)hipFreeAsync(mem, s);
+  }
+};
+
+template <class A, int K>
+int make_order(const kd_tree* t, const A* queries, uint64_t m, uint32_t flags, hipStream_t s, Order& o) {
+  if (!(flags & KD_MORTON) || m < 2) return KD_OK;
+  if (m > 0xFFFFFFFFull) return set_error(KD_ERR_INVALID, "Morton ordering supports < 2^32 queries");
+  o.s = s;
+  size_t temp = 0;
+  uint32_t *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr;
+  KD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, k0, k1, v0, v1, int(m), 0, 32, s));
+  const size_t arr = ((m * 4 + 255) / 256) * 256;
+  const size_t total = 4 * arr + temp;
+  KD_HIP(hipMallocAsync(&o.mem, total, s));
+  uint8_t* base = static_cast<uint8_t*>(o.mem);
+  k0 = reinterpret_cast<uint32_t*>(base);
+  k1 = reinterpret_cast<uint32_t*>(base + arr);
+  v0 = reinterpret_cast<uint32_t*>(base + 2 * arr);
+  v1 = reinterpret_cast<uint32_t*>(base + 3 * arr);
+  void* tmp = base + 4 * arr;
+  hipLaunchKernelGGL((dev::k_morton<A, K>), dim3(grid_for(m)), dim3(dev::kBlock), 0, s, queries, m,
+                     t->d_bbox, t->d_bbox + 4, k0, v0);
+  KD_HIP(hipGetLastError());
+  KD_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, k0, k1, v0, v1, int(m), 0, 32, s));
+  o.perm = v1;
+  return KD_OK;
+}
+
+int check_tree(const kd_tree* t) {
+  if (!t) return set_error(KD_ERR_INVALID, "tree is null");
+  return KD_OK;
+}
+
+int check_radius(double radius) {
+  if (std::isnan(radius)) return set_error(KD_ERR_INVALID, "radius is NaN");
+  return KD_OK;
+}
+
+}  // namespace
+}  // namespace kd
+
+// ====================================================================== C ABI ========
+
+struct kd_layout {
+  kd::HostLayout L;
+};
+
+extern "C" {
+
+const char* kd_last_error(void) { return kd::g_last_error.c_str(); }
+int kd_version(void) { return 1; }
+
+int kd_build_layout(int32_t dtype, int32_t dims, int32_t bucket, uint64_t n, const void* points,
+                    const uint32_t* items, int32_t threads, kd_layout** out) {
+  if (!out) return kd::set_error(KD_ERR_INVALID, "out is null");
+  try {
+    auto* l = new kd_layout;
+    int rc = kd::build_layout(l->L, dtype, dims, bucket, n, points, items, threads);
+    if (rc != KD_OK) {
+      delete l;
+      return rc;
+    }
+    *out = l;
+    return KD_OK;
+  } catch (const std::bad_alloc&) {
+    return kd::set_error(KD_ERR_NO_MEMORY, "host allocation failed while building");
+  } catch (...) {
+    return kd::set_error(KD_ERR_INVALID, "unexpected failure while building");
+  }
+}
+
+int kd_layout_desc(const kd_layout* layout, kd_tree_desc* out) {
+  if (!layout || !out) return kd::set_error(KD_ERR_INVALID, "null argument");
+  const kd::HostLayout& L = layout->L;
+  out->dtype = L.dtype;
+  out->dims = L.dims;
+  out->bucket = L.bucket;
+  out->max_stem_level = L.max_stem_level;
+  out->size = L.size;
+  out->stems_len = L.stems_len;
+  out->stems = L.stems.data();
+  out->leaf_count = L.extents.size() / 2;
+  out->leaf_extents = L.extents.data();
+  out->leaf_bytes_len = L.arena.size();
+  out->leaf_bytes = L.arena.data();
+  return KD_OK;
+}
+
+void kd_layout_free(kd_layout* layout) { delete layout; }
+
+int kd_tree_create(const kd_tree_desc* desc, uint32_t tree_flags, kd_tree** out) {
+  if (!desc || !out) return kd::set_error(KD_ERR_INVALID, "null argument");
+  if (desc->leaf_count && (!desc->leaf_extents || !desc->leaf_bytes))
+    return kd::set_error(KD_ERR_INVALID, "leaf buffers are null");
+  if (desc->stems_len && !desc->stems) return kd::set_error(KD_ERR_INVALID, "stems is null");
+  try {
+    KD_DISPATCH_AK(desc->dtype, desc->dims, return (kd::create_typed<A, KK>(desc, tree_flags, out)));
+  } catch (const std::bad_alloc&) {
+    return kd::set_error(KD_ERR_NO_MEMORY, "host allocation failed while uploading");
+  }
+  return KD_ERR_INVALID;
+}
+
+int kd_tree_destroy(kd_tree* tree) {
+  if (!tree) return KD_OK;
+  (void)hipDeviceSynchronize();
+  kd::free_tree(tree);
+  return KD_OK;
+}
+
+int kd_tree_get_info(const kd_tree* t, kd_tree_info* out) {
+  if (!t || !out) return kd::set_error(KD_ERR_INVALID, "null argument");
+  out->dtype = t->dtype;
+  out->dims = t->dims;
+  out->max_stem_level = t->max_stem_level;
+  out->has_shadow_f32 = t->shadow != nullptr;
+  out->size = t->size;
+  out->leaf_count = t->leaf_count;
+  out->max_leaf_len = t->max_leaf_len;
+  out->device_bytes = t->device_bytes;
+  for (int d = 0; d < 4; ++d) {
+    out->bbox_min[d] = t->bbox_min[d];
+    out->bbox_max[d] = t->bbox_max[d];
+  }
+  return KD_OK;
+}
+
+int kd_tree_tune(kd_tree* t, int32_t knn_variant, int32_t group_size, int32_t lds_levels) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (knn_variant < 0 || knn_variant > 2) return kd::set_error(KD_ERR_INVALID, "knn_variant must be 0, 1 or 2");
+  if (group_size != 0 && group_size != 4 && group_size != 8 && group_size != 16 && group_size != 32 &&
+      group_size != 64)
+    return kd::set_error(KD_ERR_INVALID, "group_size must be 0, 4, 8, 16, 32 or 64");
+  if (lds_levels < -1 || lds_levels > 14) return kd::set_error(KD_ERR_INVALID, "lds_levels must be in [-1, 14]");
+  t->knn_variant = knn_variant;
+  t->group_size = group_size;
+  t->lds_levels = lds_levels;
+  return KD_OK;
+}
+
+#define KD_ORDERED(a, queries_, m_, flags_, stream_)                                  \
+  hipStream_t s_ = static_cast<hipStream_t>(stream_);                                 \
+  kd::Order order_;                                                                   \
+  {                                                                                   \
+    int orc_ = KD_OK;                                                                 \
+    KD_DISPATCH_AK(t->dtype, t->dims,                                                 \
+                   orc_ = (kd::make_order<A, KK>(t, static_cast<const A*>(queries_), m_, flags_, s_, order_))); \
+    if (orc_) return orc_;                                                            \
+  }                                                                                   \
+  const kd::QueryArgs a{t, queries_, m_, order_.perm, flags_, s_};
+
+int kd_nearest_one(kd_tree* t, const void* queries, uint64_t m, uint32_t flags, uint32_t* items,
+                   void* dists, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (m == 0) return KD_OK;
+  if (!queries || !items || !dists) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_nearest_one(a, items, dists);
+}
+
+int kd_approx_nearest_one(kd_tree* t, const void* queries, uint64_t m, uint32_t flags, uint32_t* items,
+                          void* dists, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (t->dtype != KD_F32 && t->dtype != KD_F64)
+    return kd::set_error(KD_ERR_UNSUPPORTED, "approx nearest_one needs a float tree (Output == A)");
+  if (m == 0) return KD_OK;
+  if (!queries || !items || !dists) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_approx_nearest_one(a, items, dists);
+}
+
+int kd_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double radius, uint32_t flags,
+                 uint32_t* items, void* dists, uint32_t* counts, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1 (NonZeroUsize)");
+  if (k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "k above %d is not provided", kd::kMaxK);
+  if (int rc = kd::check_radius(radius)) return rc;
+  if ((flags & KD_SCREEN_F32) && !t->shadow)
+    return kd::set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
+  if (m == 0) return KD_OK;
+  if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_nearest_n(a, int(k), radius, items, dists, counts);
+}
+
+int kd_within_count(kd_tree* t, const void* queries, uint64_t m, double radius, uint32_t flags,
+                    uint64_t* offsets, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (!offsets) return kd::set_error(KD_ERR_INVALID, "offsets is null");
+  if (int rc = kd::check_radius(radius)) return rc;
+  if (m == 0) {
+    hipLaunchKernelGGL(kd::dev::k_set_u64, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), offsets,
+                       uint64_t(0));
+    KD_HIP(hipGetLastError());
+    return KD_OK;
+  }
+  if (!queries) return kd::set_error(KD_ERR_INVALID, "queries is null");
+  if (m >= 0x7FFFFFFFull) return kd::set_error(KD_ERR_UNSUPPORTED, "within batches hold < 2^31 queries");
+  KD_ORDERED(a, queries, m, flags, stream);
+  // counts into a scratch array, then an exclusive scan of m+1 entries into offsets
+  void* mem = nullptr;
+  size_t temp = 0;
+  uint64_t* cnt = nullptr;
+  KD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, cnt, offsets, int(m + 1), s_));
+  const size_t arr = ((m + 1) * 8 + 255) / 256 * 256;
+  KD_HIP(hipMallocAsync(&mem, arr + temp, s_));
+  cnt = static_cast<uint64_t*>(mem);
+  int rc = kd::launch_within_count(a, radius, cnt);
+  if (rc == KD_OK) {
+    hipLaunchKernelGGL(kd::dev::k_set_u64, dim3(1), dim3(1), 0, s_, cnt + m, uint64_t(0));
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(static_cast<uint8_t*>(mem) + arr, temp, cnt, offsets,
+                                                    int(m + 1), s_);
+    if (e != hipSuccess) rc = kd::set_error(KD_ERR_DEVICE, "scan failed: %s", hipGetErrorString(e));
+  }
+  (void)hipFreeAsync(mem, s_);
+  return rc;
+}
+
+int kd_within_fill(kd_tree* t, const void* queries, uint64_t m, double radius, uint32_t flags,
+                   const uint64_t* offsets, uint32_t* items, void* dists, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (m == 0) return KD_OK;
+  if (!queries || !offsets) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  if (int rc = kd::check_radius(radius)) return rc;
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_within_fill(a, radius, offsets, items, dists);
+}
+
+int kd_best_n_within(kd_tree* t, const void* queries, uint64_t m, double radius, uint32_t n, uint32_t flags,
+                     uint32_t* items, void* dists, uint32_t* counts, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (n == 0) return kd::set_error(KD_ERR_INVALID, "n must be >= 1 (NonZero)");
+  if (n > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "n above %d is not provided", kd::kMaxK);
+  if (int rc = kd::check_radius(radius)) return rc;
+  if (m == 0) return KD_OK;
+  if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_best_n_within(a, int(n), radius, items, dists, counts);
+}
+
+int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double radius,
+                          uint32_t flags, int32_t* chosen3, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (k == 0 || k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_INVALID, "k out of range");
+  if (!queries) return kd::set_error(KD_ERR_INVALID, "queries is null");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t ms = std::min<uint64_t>(m, uint64_t(1) << 18);
+  if (ms == 0) return KD_OK;
+  // candidate launch shapes: thread per query, and groups of the two smallest sizes
+  // holding k, each with and without LDS-pinned stems
+  int g0 = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
+  struct Cand { int variant, group, lds; };
+  std::vector<Cand> cands = {{1, 0, 0}, {1, 0, -1}, {2, g0, 0}, {2, g0, -1}};
+  if (g0 < 64) cands.push_back({2, 2 * g0, -1});
+  void* mem = nullptr;
+  const size_t osz = ms * k * (sizeof(uint32_t) + sizeof(double)) + ms * sizeof(uint32_t) + 1024;
+  KD_HIP(hipMalloc(&mem, osz));
+  uint32_t* items = static_cast<uint32_t*>(mem);
+  double* dists = reinterpret_cast<double*>(static_cast<uint8_t*>(mem) + ((ms * k * 4 + 255) / 256) * 256);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(dists + ms * k);
+  const int save_variant = t->knn_variant, save_group = t->group_size, save_lds = t->lds_levels;
+  const bool save_timing = t->timing;
+  int rc = kd_tree_set_timing(t, 1);
+  float best_ms = 1e30f;
+  Cand best = cands[0];
+  for (const Cand& c : cands) {
+    if (rc) break;
+    t->knn_variant = c.variant;
+    t->group_size = c.group;
+    t->lds_levels = c.lds;
+    float ms_c = 0.0f;
+    for (int rep = 0; rep < 2 && rc == KD_OK; ++rep) {
+      rc = kd_nearest_n(t, queries, ms, k, radius, flags, items, dists, counts, stream);
+      if (rc == KD_OK) rc = kd_last_kernel_ms(t, &ms_c);
+    }
+    if (rc == KD_OK && ms_c < best_ms) {
+      best_ms = ms_c;
+      best = c;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(mem);
+  t->timing = save_timing;
+  if (rc) {
+    t->knn_variant = save_variant;
+    t->group_size = save_group;
+    t->lds_levels = save_lds;
+    return rc;
+  }
+  t->knn_variant = best.variant;
+  t->group_size = best.group;
+  t->lds_levels = best.lds;
+  if (chosen3) {
+    chosen3[0] = best.variant;
+    chosen3[1] = best.group;
+    chosen3[2] = best.lds;
+  }
+  return KD_OK;
+}
+
+int kd_tree_set_timing(kd_tree* t, int32_t enable) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (enable && !t->ev_start) {
+    KD_HIP(hipEventCreate(&t->ev_start));
+    KD_HIP(hipEventCreate(&t->ev_stop));
+  }
+  t->timing = enable != 0;
+  t->ev_pending = false;
+  return KD_OK;
+}
+
+int kd_last_kernel_ms(kd_tree* t, float* ms) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (!ms) return kd::set_error(KD_ERR_INVALID, "ms is null");
+  if (!t->timing || !t->ev_pending) return kd::set_error(KD_ERR_INVALID, "no timed call recorded");
+  KD_HIP(hipEventSynchronize(t->ev_stop));
+  KD_HIP(hipEventElapsedTime(ms, t->ev_start, t->ev_stop));
+  return KD_OK;
+}
+
+int kd_query_stats(kd_tree* t, uint64_t* out4) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (!out4) return kd::set_error(KD_ERR_INVALID, "out is null");
+  unsigned long long h[4] = {0, 0, 0, 0};
+  KD_HIP(hipDeviceSynchronize());
+  KD_HIP(hipMemcpy(h, t->counters, sizeof(h), hipMemcpyDeviceToHost));
+  for (int j = 0; j < 4; ++j) out4[j] = h[j];
+  return KD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,800 @@
+// device.h — device-side building blocks shared by the query kernels: numeric traits,
+// the tree view, Kiddo-order traversal (thread-per-query and group-per-query forms) and
+// the result collectors.
+//
+// The traversal takes the same decisions as Kiddo's arithmetic DFS
+// (kiddo src/kd_tree/orchestrator/mod.rs:388-603): near child first, far child queued when
+// its incremental rect distance rd' = (rd - old_off^2) + new_off^2
+// (src/dist/distance_metric_core.rs:89-98) passes the current bound, and queued far children
+// re-checked against the bound of the moment when they are popped (:472-483). Kiddo keeps a
+// restore stack and a far stack; here the stack is one slot per stem level plus two bit
+// masks, because the Eytzinger path of the last leaf gives every ancestor index
+// (eytzinger/mod.rs:131-168) and an off[] entry only changes when a far child is entered,
+// so only those levels need a saved value.
+//
+// Distances are SquaredEuclidean in Kiddo's scalar order: ((0 + d0*d0) + d1*d1) + ...,
+// each product and sum rounded separately (squared_euclidean/mod.rs:36-39,
+// distance_metric_core.rs:38-40); the explicit *_rn intrinsics keep the compiler from
+// fusing them, and the build also passes -ffp-contract=off.
+//
+// Results use the canonical order (distance, then lower item): the collectors accept an
+// equal-distance candidate with a lower item, and an equal-distance subtree is only pruned
+// by an exclusive radius, so the answer does not depend on within-leaf order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kd {
+namespace dev {
+
+template <class A> struct Ax;
+template <> struct Ax<float> {
+  using O = float;
+  static __device__ __host__ constexpr float maxv() { return __builtin_huge_valf(); }
+};
+template <> struct Ax<double> {
+  using O = double;
+  static __device__ __host__ constexpr double maxv() { return __builtin_huge_val(); }
+};
+template <> struct Ax<uint16_t> {
+  using O = double;
+  static __device__ __host__ constexpr uint16_t maxv() { return 0xFFFFu; }
+};
+template <> struct Ax<uint32_t> {
+  using O = double;
+  static __device__ __host__ constexpr uint32_t maxv() { return 0xFFFFFFFFu; }
+};
+
+template <class O> __device__ __host__ constexpr O oinf();
+template <> __device__ __host__ constexpr float oinf<float>() { return __builtin_huge_valf(); }
+template <> __device__ __host__ constexpr double oinf<double>() { return __builtin_huge_val(); }
+
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+
+// |a - b| in O (Axis::saturating_dist for floats, axis.rs:124; exact for widened ints)
+template <class O> __device__ __forceinline__ O abs_diff(O a, O b) {
+  return a >= b ? sub_rn(a, b) : sub_rn(b, a);
+}
+
+// Register-resident small arrays indexed by a run-time dimension.
+template <class T, int K> __device__ __forceinline__ T pick(const T (&a)[K], int d) {
+  T r = a[0];
+#pragma unroll
+  for (int j = 1; j < K; ++j)
+    if (d == j) r = a[j];
+  return r;
+}
+template <class T, int K> __device__ __forceinline__ void put(T (&a)[K], int d, T v) {
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    if (d == j) a[j] = v;
+}
+
+constexpr int kMaxLevels = 32;
+
+template <class A, int K> struct DevTree {
+  const A* __restrict__ stems;
+  const uint32_t* __restrict__ leaf_start;
+  const A* __restrict__ coords;  // K planes of `plane` entries
+  const uint32_t* __restrict__ items;
+  const float4* __restrict__ shadow;
+  const A* lds;     // the first lds_n stems, copied into LDS by the kernel prologue
+  uint32_t lds_n;   // 0: no LDS copy
+  uint64_t plane;
+  uint32_t n;
+  int nlev;  // number of stem levels = max_stem_level + 1
+};
+
+template <class A, int K>
+__device__ __forceinline__ A stem_at(const DevTree<A, K>& t, uint32_t idx) {
+  return idx < t.lds_n ? t.lds[idx] : t.stems[idx];
+}
+
+// Copies the top stem levels into LDS (dynamic shared memory of lds_n entries). Every
+// thread of the block must call it.
+template <class A, int K> __device__ __forceinline__ void pin_stems(DevTree<A, K>& t) {
+  extern __shared__ __align__(16) unsigned char kd_lds_raw[];
+  A* lds = reinterpret_cast<A*>(kd_lds_raw);
+  for (uint32_t j = threadIdx.x; j < t.lds_n; j += blockDim.x) lds[j] = t.stems[j];
+  __syncthreads();
+  t.lds = lds;
+}
+
+// Squared distance of entry p in Kiddo's operation order.
+template <class A, int K>
+__device__ __forceinline__ typename Ax<A>::O point_dist(const DevTree<A, K>& t, uint32_t p,
+                                                        const typename Ax<A>::O (&qw)[K]) {
+  using O = typename Ax<A>::O;
+  O acc = O(0);
+#pragma unroll
+  for (int d = 0; d < K; ++d) {
+    const O c = O(t.coords[d * t.plane + p]);
+    const O df = sub_rn(c, qw[d]);
+    acc = add_rn(acc, mul_rn(df, df));
+  }
+  return acc;
+}
+
+// Per-thread traversal counters, summed per wave into the tree's counters: leaves
+// visited, entries scanned, stem nodes read, entries re-measured in FP64 (screen only).
+struct Counters {
+  uint32_t leaves = 0, points = 0, stems = 0, rescored = 0;
+};
+
+// ------------------------------------------------------------------ traversal --------
+// V supplies bound(), prune(rd), unbounded() and leaf(tree, qw, leaf_index).
+template <class A, int K, class V>
+__device__ __forceinline__ void traverse(const DevTree<A, K>& t, const A (&q)[K],
+                                         const typename Ax<A>::O (&qw)[K], V& vis, Counters& c) {
+  using O = typename Ax<A>::O;
+  if (t.n == 0) return;  // orchestrator/mod.rs:399-401
+  const int D = t.nlev;
+  O off[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) off[j] = O(0);
+  O slot[kMaxLevels];   // far rd while pending, saved off[dim] while inside the far child
+  uint32_t pending = 0, taken = 0;
+  O rd = O(0);
+  uint32_t idx = 1;
+  int level = 0, dim = 0;
+  bool check = !vis.unbounded();
+  for (;;) {
+    // descend to a leaf (orchestrator/mod.rs:520-603)
+    c.stems += uint32_t(D - level);
+    while (level < D) {
+      const A pivot = stem_at(t, idx);
+      if (pivot < Ax<A>::maxv()) {
+        const bool right = pick(q, dim) >= pivot;
+        const O qd = pick(qw, dim);
+        const O no = abs_diff(qd, O(pivot));
+        const O oo = pick(off, dim);
+        const O rdf = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+        if (!check || !(rdf > vis.bound())) {
+          slot[level] = rdf;
+          pending |= 1u << level;
+        }
+        idx = (idx << 1) | (right ? 1u : 0u);
+      } else {
+        idx <<= 1;  // sentinel stem: only the left child holds entries
+      }
+      ++level;
+      dim = (dim + 1 == K) ? 0 : dim + 1;
+    }
+    {
+      const uint32_t leaf = idx - (1u << D);
+      c.leaves += 1;
+      c.points += t.leaf_start[leaf + 1] - t.leaf_start[leaf];
+      vis.leaf(t, qw, leaf);
+    }
+    // pop frames from the deepest level upwards (orchestrator/mod.rs:450-515)
+    bool entered = false;
+    while (pending | taken) {
+      const int l = 31 - __clz(pending | taken);
+      const uint32_t bit = 1u << l;
+      const int ld = l % K;
+      if (taken & bit) {
+        put(off, ld, slot[l]);
+        taken ^= bit;
+        continue;
+      }
+      pending ^= bit;
+      const O frd = slot[l];
+      if (vis.prune(frd)) continue;
+      const A pivot = stem_at(t, idx >> (D - l));
+      c.stems += 1;
+      const O no = abs_diff(pick(qw, ld), O(pivot));
+      slot[l] = pick(off, ld);
+      taken |= bit;
+      put(off, ld, no);
+      rd = frd;
+      idx = (idx >> (D - l - 1)) ^ 1u;  // far child of the level-l ancestor
+      level = l + 1;
+      dim = level % K;
+      check = true;
+      entered = true;
+      break;
+    }
+    if (!entered) return;
+  }
+}
+
+// ------------------------------------------------------------------- visitors --------
+
+// nearest_one: best (distance, item) so far; bound = best distance
+// (nearest_one.rs:330-352). Kiddo prunes on equal and keeps the first of equal minima;
+// the canonical rule keeps equal-distance subtrees and takes the lower item.
+template <class O> struct NearestOneVis {
+  O best;
+  uint32_t item;
+  __device__ O bound() const { return best; }
+  __device__ bool prune(O rd) const { return rd > best; }
+  __device__ bool unbounded() const { return true; }
+  template <class A, int K>
+  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    for (uint32_t p = s; p < e; ++p) {
+      const O d = point_dist(t, p, qw);
+      if (d <= best) {
+        const uint32_t it = t.items[p];
+        if (d < best || it < item) {
+          best = d;
+          item = it;
+        }
+      }
+    }
+  }
+};
+
+// Sorted top-k in registers, ordered by (distance, item). Unused slots hold (+inf, ~0u).
+template <class O, int MAXK> struct TopK {
+  O d[MAXK];
+  uint32_t it[MAXK];
+  __device__ void init() {
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) {
+      d[j] = oinf<O>();
+      it[j] = 0xFFFFFFFFu;
+    }
+  }
+  static __device__ __forceinline__ bool less(O ad, uint32_t ai, O bd, uint32_t bi) {
+    return ad < bd || (ad == bd && ai < bi);
+  }
+  // inserts c, dropping the last slot
+  __device__ __forceinline__ void insert(O cd, uint32_t ci) {
+    bool lt[MAXK];
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) lt[j] = less(cd, ci, d[j], it[j]);
+#pragma unroll
+    for (int j = MAXK - 1; j > 0; --j) {
+      const bool sh = lt[j - 1];
+      const bool here = lt[j];
+      d[j] = sh ? d[j - 1] : (here ? cd : d[j]);
+      it[j] = sh ? it[j - 1] : (here ? ci : it[j]);
+    }
+    if (lt[0]) {
+      d[0] = cd;
+      it[0] = ci;
+    }
+  }
+  __device__ __forceinline__ void kth(int k, O& kd, uint32_t& kit) const {
+    if (k == MAXK) {
+      kd = d[MAXK - 1];
+      kit = it[MAXK - 1];
+      return;
+    }
+    kd = d[0];
+    kit = it[0];
+#pragma unroll
+    for (int j = 1; j < MAXK; ++j)
+      if (j == k - 1) {
+        kd = d[j];
+        kit = it[j];
+      }
+  }
+};
+
+// nearest_n / nearest_n().within (nearest_n_within.rs:441-479): bound = min(k-th distance
+// when full, radius); leaf filter `<= radius` (`<` exclusive); a full collector accepts a
+// candidate that is smaller in (distance, item) order.
+template <class O, int MAXK> struct KnnVis {
+  TopK<O, MAXK> top;
+  int k, cnt;
+  O radius;
+  bool excl;
+  O kd;
+  uint32_t kit;
+  __device__ void init(int k_, O r, bool ex) {
+    top.init();
+    k = k_;
+    cnt = 0;
+    radius = r;
+    excl = ex;
+    kd = oinf<O>();
+    kit = 0xFFFFFFFFu;
+  }
+  __device__ O bound() const { return (cnt >= k && kd < radius) ? kd : radius; }
+  __device__ bool prune(O rd) const { return rd > bound() || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  __device__ __forceinline__ void offer(O d, uint32_t it) {
+    top.insert(d, it);
+    cnt = cnt < k ? cnt + 1 : k;
+    if (cnt >= k) top.kth(k, kd, kit);
+  }
+  template <class A, int K>
+  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    for (uint32_t p = s; p < e; ++p) {
+      const O d = point_dist(t, p, qw);
+      const bool ok = excl ? d < radius : d <= radius;
+      if (!ok) continue;
+      if (cnt >= k && !(d <= kd)) continue;
+      const uint32_t it = t.items[p];
+      if (cnt >= k && !TopK<O, MAXK>::less(d, it, kd, kit)) continue;
+      offer(d, it);
+    }
+  }
+};
+
+// within (count / fill): bound = radius, prune on equal iff exclusive
+// (within_unsorted.rs:267-286).
+template <class O> struct WithinVis {
+  O radius;
+  bool excl;
+  uint64_t count;
+  uint32_t* out_items;  // null: count only
+  O* out_d;
+  uint64_t cap;
+  __device__ O bound() const { return radius; }
+  __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  template <class A, int K>
+  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    for (uint32_t p = s; p < e; ++p) {
+      const O d = point_dist(t, p, qw);
+      const bool ok = excl ? d < radius : d <= radius;
+      if (!ok) continue;
+      if (out_items && count < cap) {
+        out_items[count] = t.items[p];
+        out_d[count] = d;
+      }
+      ++count;
+    }
+  }
+};
+
+// best_n_within (best_n_within.rs:163-196, result_collection.rs:463-561): the n smallest
+// items within the radius; registers hold them sorted by item.
+template <class O, int MAXK> struct BestNVis {
+  O d[MAXK];
+  uint32_t it[MAXK];
+  int n, cnt;
+  O radius;
+  bool excl;
+  uint32_t kit;
+  __device__ void init(int n_, O r, bool ex) {
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) {
+      d[j] = oinf<O>();
+      it[j] = 0xFFFFFFFFu;
+    }
+    n = n_;
+    cnt = 0;
+    radius = r;
+    excl = ex;
+    kit = 0xFFFFFFFFu;
+  }
+  __device__ O bound() const { return radius; }
+  __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  template <class A, int K>
+  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    for (uint32_t p = s; p < e; ++p) {
+      const O dd = point_dist(t, p, qw);
+      const bool ok = excl ? dd < radius : dd <= radius;
+      if (!ok) continue;
+      const uint32_t ci = t.items[p];
+      if (cnt >= n && !(ci < kit)) continue;
+      bool lt[MAXK];
+#pragma unroll
+      for (int j = 0; j < MAXK; ++j) lt[j] = ci < it[j];
+#pragma unroll
+      for (int j = MAXK - 1; j > 0; --j) {
+        const bool sh = lt[j - 1], here = lt[j];
+        d[j] = sh ? d[j - 1] : (here ? dd : d[j]);
+        it[j] = sh ? it[j - 1] : (here ? ci : it[j]);
+      }
+      if (lt[0]) {
+        d[0] = dd;
+        it[0] = ci;
+      }
+      cnt = cnt < n ? cnt + 1 : n;
+      if (cnt >= n) {
+        kit = it[0];
+#pragma unroll
+        for (int j = 1; j < MAXK; ++j)
+          if (j == n - 1) kit = it[j];
+      }
+    }
+  }
+};
+
+// FP32-screened k-NN for f64 trees. Every leaf entry is first measured in FP32 from the
+// shadow copy; only entries whose FP32 distance is within the certified error bound of the
+// current FP64 bound are re-measured in FP64 and offered to the collector, so the result
+// is the plain kernel's. Bound derivation, with u = 2^-24 and eta = 2^-150:
+//   |fl32(p) - p| <= u|p| + eta, so each FP32 coordinate difference delta_i satisfies
+//   |delta_i - (p_i - q_i)| <= u(2+u)(|p_i| + |q_i|) + 3 eta <= eps (per query, from the
+//   largest |coordinate| of the tree and the query), hence ||delta|| <= ||t|| + sqrt(K) eps.
+//   The FP32 sum of squares of non-negative terms is at most ||delta||^2 (1+u)^K, and the
+//   exact distance is at most the FP64 one times (1 + 1e-15). So FP64 dist <= T implies
+//   FP32 dist <= (sqrt(T (1+1e-15)) + sqrt(K) eps)^2 (1 + 4u) + 1e-44, rounded up.
+__device__ __forceinline__ float screen_threshold(double T, double eps_k) {
+  if (!(T < __builtin_huge_val())) return __builtin_huge_valf();
+  const double r = sqrt(T * (1.0 + 1e-15)) + eps_k;
+  const double tt = r * r * (1.0 + 4.0 * 0x1p-24) + 1e-44;
+  float f = (float)tt;
+  if ((double)f < tt) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f > 0)
+  return f;
+}
+
+template <int MAXK> struct ScreenKnnVis {
+  KnnVis<double, MAXK> v;
+  float qf[3];
+  double eps_k;
+  float thr;
+  uint32_t rescored;
+  __device__ double bound() const { return v.bound(); }
+  __device__ bool prune(double rd) const { return v.prune(rd); }
+  __device__ bool unbounded() const { return v.unbounded(); }
+  template <int K>
+  __device__ void leaf(const DevTree<double, K>& t, const double (&qw)[K], uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    for (uint32_t p = s; p < e; ++p) {
+      const float4 sh = t.shadow[p];
+      float acc;
+      {
+        const float dx = sub_rn(sh.x, qf[0]);
+        acc = mul_rn(dx, dx);
+        const float dy = sub_rn(sh.y, qf[1]);
+        acc = add_rn(acc, mul_rn(dy, dy));
+        if (K > 2) {
+          const float dz = sub_rn(sh.z, qf[2]);
+          acc = add_rn(acc, mul_rn(dz, dz));
+        }
+      }
+      if (acc > thr) continue;
+      ++rescored;
+      const double d = point_dist(t, p, qw);
+      const bool ok = v.excl ? d < v.radius : d <= v.radius;
+      if (!ok) continue;
+      const uint32_t it = __float_as_uint(sh.w);
+      if (v.cnt >= v.k && !TopK<double, MAXK>::less(d, it, v.kd, v.kit)) continue;
+      v.offer(d, it);
+      thr = screen_threshold(v.bound(), eps_k);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------- kernels --------
+
+// Sums the wave's counters and adds them to the global ones with one atomic each.
+// Every lane of the wave must reach this call.
+__device__ __forceinline__ void flush_counters(const Counters& c, unsigned long long* __restrict__ out) {
+  if (!out) return;
+  unsigned long long v[4] = {c.leaves, c.points, c.stems, c.rescored};
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    for (int o = 32; o > 0; o >>= 1) v[j] += __shfl_xor(v[j], o);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (v[j]) atomicAdd(&out[j], v[j]);
+  }
+}
+
+template <class A, int K>
+__device__ __forceinline__ void load_query(const A* __restrict__ queries, uint64_t qi, A (&q)[K],
+                                           typename Ax<A>::O (&qw)[K]) {
+  using O = typename Ax<A>::O;
+#pragma unroll
+  for (int d = 0; d < K; ++d) {
+    q[d] = queries[qi * K + d];
+    qw[d] = O(q[d]);
+  }
+}
+
+constexpr int kBlock = 256;
+
+// Kernel prologue shared by every family: one thread per query, queries optionally
+// visited in a permuted (Morton) order; results always land at the query's own index.
+#define KD_QUERY_PROLOGUE                                             \
+  using O = typename Ax<A>::O;                                        \
+  pin_stems(t);                                                       \
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;     \
+  const bool active = i < m;                                          \
+  const uint64_t qi = active ? (order ? order[i] : i) : 0;            \
+  A q[K];                                                             \
+  O qw[K];                                                            \
+  if (active) load_query<A, K>(queries, qi, q, qw);                   \
+  Counters cnt;
+
+// ------------------------------------------------------------ group-per-query --------
+// G lanes (G divides 64) answer one query together. Control flow is uniform inside a
+// group: every lane walks the same traversal, the lanes split each leaf's entries between
+// them, and lane gl keeps
+//   * slot gl of the sorted top-k (k <= G), so inserting a candidate is one shift across
+//     the group's lanes;
+//   * the DFS stack entries of levels gl, gl + G, ... in registers.
+
+template <int G> struct GroupStack {
+  static constexpr int R = (kMaxLevels + G - 1) / G;
+};
+
+template <class O, int G>
+__device__ __forceinline__ void gs_write(O (&reg)[GroupStack<G>::R], int gl, int l, O v) {
+  const int r = l / G, owner = l % G;
+#pragma unroll
+  for (int j = 0; j < GroupStack<G>::R; ++j)
+    if (j == r && gl == owner) reg[j] = v;
+}
+
+template <class O, int G>
+__device__ __forceinline__ O gs_read(const O (&reg)[GroupStack<G>::R], int l) {
+  const int r = l / G;
+  O v = reg[0];
+#pragma unroll
+  for (int j = 1; j < GroupStack<G>::R; ++j)
+    if (j == r) v = reg[j];
+  return __shfl(v, l % G, G);
+}
+
+// The traversal of `traverse` with the stack spread over the group's lanes. V::leaf is
+// called by every lane of the group.
+template <class A, int K, int G, class V>
+__device__ __forceinline__ void group_traverse(const DevTree<A, K>& t, const A (&q)[K],
+                                               const typename Ax<A>::O (&qw)[K], V& vis,
+                                               Counters& c, int gl) {
+  using O = typename Ax<A>::O;
+  if (t.n == 0) return;
+  const int D = t.nlev;
+  O off[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) off[j] = O(0);
+  O reg[GroupStack<G>::R];
+#pragma unroll
+  for (int j = 0; j < GroupStack<G>::R; ++j) reg[j] = O(0);
+  uint32_t pending = 0, taken = 0;
+  O rd = O(0);
+  uint32_t idx = 1;
+  int level = 0, dim = 0;
+  bool check = !vis.unbounded();
+  for (;;) {
+    c.stems += uint32_t(D - level);
+    while (level < D) {
+      const A pivot = stem_at(t, idx);
+      if (pivot < Ax<A>::maxv()) {
+        const bool right = pick(q, dim) >= pivot;
+        const O no = abs_diff(pick(qw, dim), O(pivot));
+        const O oo = pick(off, dim);
+        const O rdf = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+        if (!check || !(rdf > vis.bound())) {
+          gs_write<O, G>(reg, gl, level, rdf);
+          pending |= 1u << level;
+        }
+        idx = (idx << 1) | (right ? 1u : 0u);
+      } else {
+        idx <<= 1;
+      }
+      ++level;
+      dim = (dim + 1 == K) ? 0 : dim + 1;
+    }
+    {
+      const uint32_t leaf = idx - (1u << D);
+      c.leaves += 1;
+      c.points += t.leaf_start[leaf + 1] - t.leaf_start[leaf];
+      vis.leaf(t, qw, leaf);
+    }
+    bool entered = false;
+    while (pending | taken) {
+      const int l = 31 - __clz(pending | taken);
+      const uint32_t bit = 1u << l;
+      const int ld = l % K;
+      if (taken & bit) {
+        put(off, ld, gs_read<O, G>(reg, l));
+        taken ^= bit;
+        continue;
+      }
+      pending ^= bit;
+      const O frd = gs_read<O, G>(reg, l);
+      if (vis.prune(frd)) continue;
+      const A pivot = stem_at(t, idx >> (D - l));
+      c.stems += 1;
+      const O no = abs_diff(pick(qw, ld), O(pivot));
+      gs_write<O, G>(reg, gl, l, pick(off, ld));
+      taken |= bit;
+      put(off, ld, no);
+      rd = frd;
+      idx = (idx >> (D - l - 1)) ^ 1u;
+      level = l + 1;
+      dim = level % K;
+      check = true;
+      entered = true;
+      break;
+    }
+    if (!entered) return;
+  }
+}
+
+// Group k-NN collector (k <= G). SCREEN: f64 trees measured first in FP32 from the shadow
+// against the certified bound of screen_threshold, exact FP64 only for survivors.
+template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {
+  using O = typename Ax<A>::O;
+  O sd;          // this lane's slot of the sorted top-k
+  uint32_t si;
+  int k, cnt, gl;
+  O radius;
+  bool excl;
+  O kd;          // k-th (last kept) entry once full, group-uniform
+  uint32_t kit;
+  float qf[3];   // SCREEN: the query in FP32
+  double eps_k;  // SCREEN: sqrt(K) * per-query coordinate-difference error bound
+  float thr;     // SCREEN: FP32 threshold certified to admit every FP64 candidate
+  uint32_t rescored;
+
+  __device__ void init(int k_, O r, bool ex, int gl_) {
+    sd = oinf<O>();
+    si = 0xFFFFFFFFu;
+    k = k_;
+    cnt = 0;
+    gl = gl_;
+    radius = r;
+    excl = ex;
+    kd = oinf<O>();
+    kit = 0xFFFFFFFFu;
+    thr = __builtin_huge_valf();
+    rescored = 0;
+  }
+  __device__ O bound() const { return (cnt >= k && kd < radius) ? kd : radius; }
+  __device__ bool prune(O rd) const { return rd > bound() || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  static __device__ __forceinline__ bool less(O ad, uint32_t ai, O bd, uint32_t bi) {
+    return ad < bd || (ad == bd && ai < bi);
+  }
+  __device__ __forceinline__ void insert(O cd, uint32_t ci) {
+    const bool lt = gl < k && less(cd, ci, sd, si);
+    const O pd = __shfl_up(sd, 1, G);
+    const uint32_t pi = __shfl_up(si, 1, G);
+    // every lane of the group must execute the shuffle; lane 0 then drops what it read
+    int plt = __shfl_up(int(lt), 1, G);
+    if (gl == 0) plt = 0;
+    if (gl < k) {
+      if (plt) {
+        sd = pd;
+        si = pi;
+      } else if (lt) {
+        sd = cd;
+        si = ci;
+      }
+    }
+    cnt = cnt < k ? cnt + 1 : k;
+    if (cnt >= k) {
+      kd = __shfl(sd, k - 1, G);
+      kit = __shfl(si, k - 1, G);
+      if (SCREEN) thr = screen_threshold(double(bound()), eps_k);
+    }
+  }
+  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    const int base = (threadIdx.x & 63) & ~(G - 1);
+    for (uint32_t c0 = s; c0 < e; c0 += G) {
+      const uint32_t p = c0 + uint32_t(gl);
+      O d = oinf<O>();
+      uint32_t it = 0xFFFFFFFFu;
+      bool ok = false;
+      if (p < e) {
+        if constexpr (SCREEN) {
+          const float4 sh = t.shadow[p];
+          const float dx = sub_rn(sh.x, qf[0]);
+          float acc = mul_rn(dx, dx);
+          const float dy = sub_rn(sh.y, qf[1]);
+          acc = add_rn(acc, mul_rn(dy, dy));
+          if (K > 2) {
+            const float dz = sub_rn(sh.z, qf[2]);
+            acc = add_rn(acc, mul_rn(dz, dz));
+          }
+          if (!(acc > thr)) {
+            ++rescored;
+            d = point_dist(t, p, qw);
+            it = __float_as_uint(sh.w);
+            ok = true;
+          }
+        } else {
+          d = point_dist(t, p, qw);
+          ok = true;
+        }
+        ok = ok && (excl ? d < radius : d <= radius) && (cnt < k || d <= kd);
+        if (ok) {
+          if constexpr (!SCREEN) it = t.items[p];
+          ok = cnt < k || less(d, it, kd, kit);
+        }
+      }
+      uint32_t mask = uint32_t((__ballot(ok) >> base) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull)));
+      while (mask) {
+        const int src = __ffs(mask) - 1;
+        mask &= mask - 1u;
+        const O cd = __shfl(d, src, G);
+        const uint32_t ci = __shfl(it, src, G);
+        if (cnt >= k && !less(cd, ci, kd, kit)) continue;
+        insert(cd, ci);
+      }
+    }
+  }
+};
+
+// Sorts one query's segment by (distance, item): insertion sort for short segments,
+// heapsort otherwise (in place, global memory).
+template <class O>
+__device__ void sort_segment(uint32_t* __restrict__ it, O* __restrict__ d, uint64_t len) {
+  auto less = [](O ad, uint32_t ai, O bd, uint32_t bi) { return ad < bd || (ad == bd && ai < bi); };
+  if (len <= 48) {
+    for (uint64_t a = 1; a < len; ++a) {
+      const O kd = d[a];
+      const uint32_t ki = it[a];
+      uint64_t b = a;
+      while (b > 0 && less(kd, ki, d[b - 1], it[b - 1])) {
+        d[b] = d[b - 1];
+        it[b] = it[b - 1];
+        --b;
+      }
+      d[b] = kd;
+      it[b] = ki;
+    }
+    return;
+  }
+  auto sift = [&](uint64_t root, uint64_t end) {
+    for (;;) {
+      uint64_t c = 2 * root + 1;
+      if (c >= end) return;
+      if (c + 1 < end && less(d[c], it[c], d[c + 1], it[c + 1])) ++c;
+      if (!less(d[root], it[root], d[c], it[c])) return;
+      const O td = d[root];
+      d[root] = d[c];
+      d[c] = td;
+      const uint32_t ti = it[root];
+      it[root] = it[c];
+      it[c] = ti;
+      root = c;
+    }
+  };
+  for (uint64_t s = len / 2; s-- > 0;) sift(s, len);
+  for (uint64_t e = len - 1; e > 0; --e) {
+    const O td = d[0];
+    d[0] = d[e];
+    d[e] = td;
+    const uint32_t ti = it[0];
+    it[0] = it[e];
+    it[e] = ti;
+    sift(0, e);
+  }
+}
+
+// Morton keys: each coordinate scaled into [0, 2^bits) over the tree's bounding box and
+// bit-interleaved, so that consecutive queries share traversal paths and leaves.
+template <class A, int K>
+__global__ __launch_bounds__(kBlock) void k_morton(const A* __restrict__ queries, uint64_t m,
+                                                   const double* __restrict__ lo,
+                                                   const double* __restrict__ scale,
+                                                   uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ ids) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= m) return;
+  constexpr int bits = 32 / K;
+  const uint32_t cap = (1u << bits) - 1u;
+  uint32_t key = 0;
+  uint32_t c[K];
+#pragma unroll
+  for (int d = 0; d < K; ++d) {
+    double x = (double(queries[i * K + d]) - lo[d]) * scale[d];
+    x = x > 0.0 ? x : 0.0;  // also maps NaN to 0
+    c[d] = x >= double(cap) ? cap : uint32_t(x);
+  }
+#pragma unroll
+  for (int b = bits - 1; b >= 0; --b)
+#pragma unroll
+    for (int d = 0; d < K; ++d) key = (key << 1) | ((c[d] >> b) & 1u);
+  keys[i] = key;
+  ids[i] = uint32_t(i);
+}
+
+
+}  // namespace dev
+}  // namespace kd

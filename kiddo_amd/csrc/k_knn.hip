@@ -1,0 +1,222 @@
+// k_knn.hip — k-NN kernels: nearest_n / nearest_n().within
+// (kiddo query/nearest_n.rs:21-38, nearest_n_within.rs:127-179, :441-479).
+//
+// Two kernel forms, chosen per call (kd_tree_tune, or automatically):
+//   * thread per query: one lane walks the DFS and keeps the top-k in registers;
+//   * group per query: G lanes share one query, split each leaf's entries, and keep the
+//     top-k spread one slot per lane (device.h group_traverse / GroupKnnVis).
+// Each form has an FP32-screened variant for f64 trees (KD_SCREEN_F32).
+
+#include "tree.h"
+
+namespace kd {
+namespace dev {
+
+template <class A, int K, int MAXK>
+__global__ __launch_bounds__(kBlock) void k_nearest_n(DevTree<A, K> t, const A* __restrict__ queries,
+                                                      const uint32_t* __restrict__ order, uint64_t m,
+                                                      int k, typename Ax<A>::O radius, int excl,
+                                                      uint32_t* __restrict__ out_items,
+                                                      typename Ax<A>::O* __restrict__ out_d,
+                                                      uint32_t* __restrict__ out_counts,
+                                                      unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    KnnVis<O, MAXK> v;
+    v.init(k, radius, excl != 0);
+    traverse(t, q, qw, v, cnt);
+    uint32_t* ri = out_items + qi * uint64_t(k);
+    O* rdd = out_d + qi * uint64_t(k);
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j)
+      if (j < k) {
+        ri[j] = j < v.cnt ? v.top.it[j] : 0xFFFFFFFFu;
+        rdd[j] = j < v.cnt ? v.top.d[j] : oinf<O>();
+      }
+    out_counts[qi] = uint32_t(v.cnt);
+  }
+  flush_counters(cnt, stats);
+}
+
+// Per-query FP32 screen set-up: the query rounded to FP32, and sqrt(K) times the bound on
+// the error of an FP32 coordinate difference (see screen_threshold).
+template <int K>
+__device__ __forceinline__ void screen_setup(const double (&q)[K], const double* __restrict__ max_abs,
+                                             float (&qf)[3], double& eps_k) {
+  double amax = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) qf[d] = 0.0f;
+#pragma unroll
+  for (int d = 0; d < K; ++d) {
+    qf[d] = (float)q[d];
+    amax = fmax(amax, max_abs[d] + fabs(q[d]));
+  }
+  const double u = 0x1p-24;
+  const double eps = u * (2.0 + u) * amax + 3.0 * 0x1p-150;
+  eps_k = sqrt((double)K) * eps * (1.0 + 1e-6);
+}
+
+template <int K, int MAXK>
+__global__ __launch_bounds__(kBlock) void k_nearest_n_screen(
+    DevTree<double, K> t, const double* __restrict__ queries, const uint32_t* __restrict__ order,
+    uint64_t m, int k, double radius, int excl, const double* __restrict__ max_abs,
+    uint32_t* __restrict__ out_items, double* __restrict__ out_d, uint32_t* __restrict__ out_counts,
+    unsigned long long* __restrict__ stats) {
+  using A = double;
+  KD_QUERY_PROLOGUE
+  if (active) {
+    ScreenKnnVis<MAXK> v;
+    v.v.init(k, radius, excl != 0);
+    screen_setup<K>(q, max_abs, v.qf, v.eps_k);
+    v.thr = screen_threshold(v.v.bound(), v.eps_k);
+    v.rescored = 0;
+    traverse(t, q, qw, v, cnt);
+    uint32_t* ri = out_items + qi * uint64_t(k);
+    double* rdd = out_d + qi * uint64_t(k);
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j)
+      if (j < k) {
+        ri[j] = j < v.v.cnt ? v.v.top.it[j] : 0xFFFFFFFFu;
+        rdd[j] = j < v.v.cnt ? v.v.top.d[j] : __builtin_huge_val();
+      }
+    out_counts[qi] = uint32_t(v.v.cnt);
+    cnt.rescored = v.rescored;
+  }
+  flush_counters(cnt, stats);
+}
+
+template <class A, int K, int G, bool SCREEN>
+__global__ __launch_bounds__(kBlock) void k_knn_group(DevTree<A, K> t, const A* __restrict__ queries,
+                                                      const uint32_t* __restrict__ order, uint64_t m,
+                                                      int k, typename Ax<A>::O radius, int excl,
+                                                      const double* __restrict__ max_abs,
+                                                      uint32_t* __restrict__ out_items,
+                                                      typename Ax<A>::O* __restrict__ out_d,
+                                                      uint32_t* __restrict__ out_counts,
+                                                      unsigned long long* __restrict__ stats) {
+  using O = typename Ax<A>::O;
+  pin_stems(t);
+  const uint64_t gid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / G;
+  const int gl = int(threadIdx.x) & (G - 1);
+  const bool active = gid < m;  // uniform within a group
+  const uint64_t qi = active ? (order ? order[gid] : gid) : 0;
+  A q[K];
+  O qw[K];
+  Counters cnt;
+  if (active) {
+    load_query<A, K>(queries, qi, q, qw);
+    GroupKnnVis<A, K, G, SCREEN> v;
+    v.init(k, radius, excl != 0, gl);
+    if constexpr (SCREEN) {
+      if constexpr (std::is_same<A, double>::value && K <= 3) {
+        screen_setup<K>(q, max_abs, v.qf, v.eps_k);
+        v.thr = screen_threshold(double(v.bound()), v.eps_k);
+      }
+    }
+    group_traverse<A, K, G>(t, q, qw, v, cnt, gl);
+    if (gl < k) {
+      out_items[qi * uint64_t(k) + gl] = gl < v.cnt ? v.si : 0xFFFFFFFFu;
+      out_d[qi * uint64_t(k) + gl] = gl < v.cnt ? v.sd : oinf<O>();
+    }
+    if (gl == 0) out_counts[qi] = uint32_t(v.cnt);
+    cnt.rescored = v.rescored;
+    if (gl != 0) cnt.leaves = cnt.points = cnt.stems = 0;  // counted once per group
+  }
+  flush_counters(cnt, stats);
+}
+
+}  // namespace dev
+
+namespace {
+
+template <class A, int K, int MAXK>
+int launch_thread(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts) {
+  using O = typename dev::Ax<A>::O;
+  kd_tree* t = a.t;
+  const int excl = (a.flags & KD_EXCLUSIVE) ? 1 : 0;
+  const A* q = static_cast<const A*>(a.queries);
+  auto dt = dev_tree<A, K>(t);
+  const size_t lds = size_t(dt.lds_n) * sizeof(A);
+  reset_counters(t, a.s);
+  KernelTimer kt(t, a.s);
+  if constexpr (std::is_same<A, double>::value && K <= 3) {
+    if (a.flags & KD_SCREEN_F32) {
+      hipLaunchKernelGGL((dev::k_nearest_n_screen<K, MAXK>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds, a.s,
+                         dt, q, a.order, a.m, k, radius, excl, t->d_max_abs, items,
+                         static_cast<double*>(dists), counts, t->counters);
+      KD_HIP(hipGetLastError());
+      return KD_OK;
+    }
+  }
+  hipLaunchKernelGGL((dev::k_nearest_n<A, K, MAXK>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds, a.s, dt,
+                     q, a.order, a.m, k, O(radius), excl, items, static_cast<O*>(dists), counts, t->counters);
+  KD_HIP(hipGetLastError());
+  return KD_OK;
+}
+
+template <class A, int K, int G>
+int launch_group(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts) {
+  using O = typename dev::Ax<A>::O;
+  kd_tree* t = a.t;
+  const int excl = (a.flags & KD_EXCLUSIVE) ? 1 : 0;
+  const A* q = static_cast<const A*>(a.queries);
+  auto dt = dev_tree<A, K>(t);
+  const size_t lds = size_t(dt.lds_n) * sizeof(A);
+  const uint32_t grid = grid_for(a.m * G);
+  reset_counters(t, a.s);
+  KernelTimer kt(t, a.s);
+  if (a.flags & KD_SCREEN_F32) {
+    if constexpr (std::is_same<A, double>::value && K <= 3) {
+      hipLaunchKernelGGL((dev::k_knn_group<A, K, G, true>), dim3(grid), dim3(dev::kBlock), lds, a.s, dt, q,
+                         a.order, a.m, k, O(radius), excl, t->d_max_abs, items, static_cast<O*>(dists), counts,
+                         t->counters);
+    }
+  } else {
+    hipLaunchKernelGGL((dev::k_knn_group<A, K, G, false>), dim3(grid), dim3(dev::kBlock), lds, a.s, dt, q,
+                       a.order, a.m, k, O(radius), excl, t->d_max_abs, items, static_cast<O*>(dists), counts,
+                       t->counters);
+  }
+  KD_HIP(hipGetLastError());
+  return KD_OK;
+}
+
+template <class A, int K>
+int launch_typed(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts) {
+  kd_tree* t = a.t;
+  if (a.flags & KD_SCREEN_F32) {
+    if (!(std::is_same<A, double>::value && K <= 3))
+      return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 applies to f64 trees with dims <= 3");
+    if (!t->shadow)
+      return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
+  }
+  // group form: k must fit in one slot per lane
+  int g = t->group_size;
+  if (g == 0) g = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
+  const bool group = t->knn_variant == 2 && g >= k;
+  if (group) {
+    switch (g) {
+      case 4: return launch_group<A, K, 4>(a, k, radius, items, dists, counts);
+      case 8: return launch_group<A, K, 8>(a, k, radius, items, dists, counts);
+      case 16: return launch_group<A, K, 16>(a, k, radius, items, dists, counts);
+      case 32: return launch_group<A, K, 32>(a, k, radius, items, dists, counts);
+      case 64: return launch_group<A, K, 64>(a, k, radius, items, dists, counts);
+      default: return set_error(KD_ERR_INVALID, "group size %d is not one of 4, 8, 16, 32, 64", g);
+    }
+  }
+  if (k <= 1) return launch_thread<A, K, 1>(a, k, radius, items, dists, counts);
+  if (k <= 4) return launch_thread<A, K, 4>(a, k, radius, items, dists, counts);
+  if (k <= 8) return launch_thread<A, K, 8>(a, k, radius, items, dists, counts);
+  if (k <= 16) return launch_thread<A, K, 16>(a, k, radius, items, dists, counts);
+  if (k <= 32) return launch_thread<A, K, 32>(a, k, radius, items, dists, counts);
+  return launch_thread<A, K, 64>(a, k, radius, items, dists, counts);
+}
+
+}  // namespace
+
+int launch_nearest_n(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists,
+                     uint32_t* counts) {
+  KD_DISPATCH_AK(a.t->dtype, a.t->dims, return (launch_typed<A, KK>(a, k, radius, items, dists, counts)));
+  return KD_ERR_UNSUPPORTED;
+}
+
+}  // namespace kd

@@ -102,10 +102,14 @@ def test_families_match_oracle(ka, oracle_mod, cuda, kind, n, K, dtype, B):
         assert_same_bits(gi, oi, f"nearest_one items {tag}")
         assert_same_bits(gd, od, f"nearest_one dists {tag}")
         for k in (1, 5, 16):
-            res_g = G.nearest_n(gt, q, k)
             res_o = O.nearest_n(ot, q, k)
-            for a, b, nm in zip(res_g, res_o, ("items", "dists", "counts")):
-                assert_same_bits(a, b, f"nearest_n({k}) {nm} {tag}")
+            # thread-per-query, group-per-query (several group sizes) and no LDS stems
+            for variant, gsize, lds in ((1, 0, -1), (2, 0, -1), (2, 32, 3), (1, 0, 0)):
+                gt.tune(variant, gsize, lds)
+                res_g = G.nearest_n(gt, q, k)
+                for a, b, nm in zip(res_g, res_o, ("items", "dists", "counts")):
+                    assert_same_bits(a, b, f"nearest_n({k}) {nm} {tag} variant={variant}/{gsize}/{lds}")
+            gt.tune(0, 0, -1)
         for excl in (False, True):
             res_g = G.nearest_n(gt, q, 8, r, excl)
             res_o = O.nearest_n(ot, q, 8, r, excl)
@@ -174,11 +178,13 @@ def test_screen_f32_equals_plain_f64(ka, oracle_mod, cuda):
     ot = O.build(pts, 32)
     for k in (1, 8, 16):
         ref = O.nearest_n(ot, q, k)
-        for flags in (ka.KD_SCREEN_F32, ka.KD_SCREEN_F32 | ka.KD_MORTON):
-            r = shadow.nearest_n_raw(q, k, flags=flags)
-            got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
-            for a, b in zip(got, ref):
-                assert_same_bits(a, b, f"screen k={k} flags={flags}")
+        for variant in (1, 2):
+            shadow.tune(variant, 0, -1)
+            for flags in (ka.KD_SCREEN_F32, ka.KD_SCREEN_F32 | ka.KD_MORTON):
+                r = shadow.nearest_n_raw(q, k, flags=flags)
+                got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
+                for a, b in zip(got, ref):
+                    assert_same_bits(a, b, f"screen k={k} flags={flags} variant={variant}")
         screened, rescored = shadow.screen_stats()
         assert 0 < rescored < screened
         r = plain.nearest_n_raw(q, k)
@@ -230,12 +236,15 @@ def test_traversal_visits_match_oracle(ka, oracle_mod, cuda):
     q = rng.random((5000, 3))
     t = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
     ot = oracle_mod.Tree.build(pts, 32)
-    for flags in (0, ka.KD_MORTON, ka.KD_SCREEN_F32 | ka.KD_MORTON):
-        t.nearest_n_raw(q, 16, flags=flags)
-        st = t.query_stats()
-        _, _, _, ost = ot.nearest_n(q, 16, stats=True)
-        assert st["leaves"] == int(ost[0]), (flags, st, ost)
-        assert st["points"] == int(ost[1]), (flags, st, ost)
+    _, _, _, ost = ot.nearest_n(q, 16, stats=True)
+    for variant in (1, 2):
+        t.tune(variant, 0, -1)
+        for flags in (0, ka.KD_MORTON, ka.KD_SCREEN_F32 | ka.KD_MORTON):
+            t.nearest_n_raw(q, 16, flags=flags)
+            st = t.query_stats()
+            assert st["leaves"] == int(ost[0]), (variant, flags, st, ost)
+            assert st["points"] == int(ost[1]), (variant, flags, st, ost)
+    t.tune(0, 0, -1)
     t.nearest_one_raw(q)
     st = t.query_stats()
     _, _, ost = ot.nearest_one(q, stats=True)
