@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
+    ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -70,6 +73,17 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+
+    if args.copy_probe:
+        # A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it scales the
+        # same counters of the query kernels into bytes.
+        src = torch.ones(args.copy_probe // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 3}), flush=True)
+        return
 
     t0 = time.time()
     pts = uniform_with_duplicates(args.points, args.dup_frac, seed=args.seed)
@@ -94,7 +108,7 @@ def main():
 
     # plain: one thread per query, queries in input order, no LDS stems. screened: the
     # launch shape the autotuner picks on a sample of the batch.
-    tuning = {"plain": dict(variant=1, group_size=0, lds_levels=0)}
+    tuning = {"plain": dict(variant=1, group_size=0, lds_levels=0, waves=0)}
     if "screened" in variants:
         tuning["screened"] = tree.autotune_nearest_n(q, k, math.inf, flags["screened"])
         log(f"autotuned screened launch: {tuning['screened']}")
@@ -104,7 +118,7 @@ def main():
 
     results = {}
     for name in variants:
-        tree.tune(tuning[name]["variant"], tuning[name]["group_size"], tuning[name]["lds_levels"])
+        tree.tune(**{("knn_variant" if a == "variant" else a): v for a, v in tuning[name].items()})
         for _ in range(args.warmup):
             run(name)
         torch.cuda.synchronize()
@@ -130,9 +144,9 @@ def main():
     head = "screened" if "screened" in results else "plain"
 
     # latency of fixed-size sub-batches (screened path)
-    tree.tune(tuning[head]["variant"], tuning[head]["group_size"], tuning[head]["lds_levels"])
+    tree.tune(**{("knn_variant" if a == "variant" else a): v for a, v in tuning[head].items()})
     lb = min(args.latency_batch, m)
-    nb = m // lb
+    nb = 0 if args.no_latency else m // lb
     sub = ka.NearestN(torch.empty((lb, k), dtype=torch.uint32, device=dev),
                       torch.empty((lb, k), dtype=torch.float64, device=dev),
                       torch.empty(lb, dtype=torch.uint32, device=dev))

@@ -113,13 +113,17 @@ int kd_tree_get_info(const kd_tree* tree, kd_tree_info* out);
  * query (group_size 0 = smallest of 4/8/16/32/64 that holds k). lds_levels: top stem
  * levels copied into LDS per block (-1 automatic, 0 none, at most 14). */
 int kd_tree_tune(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_t lds_levels);
+/* As kd_tree_tune, plus the minimum waves per SIMD the thread-per-query k-NN kernel is
+ * compiled for (0 = default 3; 4..6 trade registers for occupancy). */
+int kd_tree_tune2(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_t lds_levels,
+                  int32_t waves);
 
 /* Autotuner: times each candidate launch shape (thread vs group per query, group size,
- * LDS stem levels) on up to 2^18 of the given queries with the given k/radius/flags, keeps
- * the fastest in the tree (as kd_tree_tune would) and reports it in chosen3 = {variant,
- * group_size, lds_levels}. Synchronous. */
+ * LDS stem levels, occupancy floor) on up to 2^18 of the given queries with the given
+ * k/radius/flags, keeps the fastest in the tree (as kd_tree_tune2 would) and reports it in
+ * chosen4 = {variant, group_size, lds_levels, waves}. Synchronous. */
 int kd_autotune_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k,
-                          double radius, uint32_t flags, int32_t* chosen3, void* stream);
+                          double radius, uint32_t flags, int32_t* chosen4, void* stream);
 
 /* --------------------------------------------------------------------- queries ---- */
 /* tree.query(q).nearest_one::<SquaredEuclidean>().execute()  (builder.rs:2008-2017;

@@ -75,6 +75,7 @@ SIGNATURES = {
     "kd_tree_destroy": ([_vp], C.c_int),
     "kd_tree_get_info": ([_vp, C.POINTER(TreeInfo)], C.c_int),
     "kd_tree_tune": ([_vp, _i32, _i32, _i32], C.c_int),
+    "kd_tree_tune2": ([_vp, _i32, _i32, _i32, _i32], C.c_int),
     "kd_autotune_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp], C.c_int),
     "kd_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_approx_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
@@ -288,18 +289,20 @@ class ImmutableKdTree:
         _check(lib().kd_query_stats(self._h, _np_ptr(out)), "kd_query_stats")
         return dict(zip(("leaves", "points", "stems", "rescored"), (int(x) for x in out)))
 
-    def tune(self, knn_variant: int = 0, group_size: int = 0, lds_levels: int = -1):
-        """Kernel form for k-NN (0 auto, 1 thread/query, 2 group/query) and LDS stem levels."""
-        _check(lib().kd_tree_tune(self._h, knn_variant, group_size, lds_levels), "kd_tree_tune")
+    def tune(self, knn_variant: int = 0, group_size: int = 0, lds_levels: int = -1, waves: int = 0):
+        """Kernel form for k-NN (0 auto, 1 thread/query, 2 group/query), LDS stem levels and
+        the occupancy floor of the thread-per-query kernel."""
+        _check(lib().kd_tree_tune2(self._h, knn_variant, group_size, lds_levels, waves),
+               "kd_tree_tune2")
 
     def autotune_nearest_n(self, q, k, radius=math.inf, flags=0, stream=None):
         """Times the candidate launch shapes on a sample of q and keeps the fastest."""
         q = self._queries(q)
-        chosen = np.zeros(3, np.int32)
+        chosen = np.zeros(4, np.int32)
         _check(lib().kd_autotune_nearest_n(self._h, q.data_ptr(), q.shape[0], k, float(radius),
                                            flags, _np_ptr(chosen), _stream_ptr(stream)),
                "kd_autotune_nearest_n")
-        return dict(zip(("variant", "group_size", "lds_levels"), (int(x) for x in chosen)))
+        return dict(zip(("variant", "group_size", "lds_levels", "waves"), (int(x) for x in chosen)))
 
     def set_timing(self, enable: bool = True):
         _check(lib().kd_tree_set_timing(self._h, int(enable)), "kd_tree_set_timing")

@@ -396,6 +396,11 @@ int kd_tree_get_info(const kd_tree* t, kd_tree_info* out) {
 }
 
 int kd_tree_tune(kd_tree* t, int32_t knn_variant, int32_t group_size, int32_t lds_levels) {
+  return kd_tree_tune2(t, knn_variant, group_size, lds_levels, 0);
+}
+
+int kd_tree_tune2(kd_tree* t, int32_t knn_variant, int32_t group_size, int32_t lds_levels, int32_t waves) {
+  if (waves != 0 && (waves < 3 || waves > 6)) return kd::set_error(KD_ERR_INVALID, "waves must be 0 or 3..6");
   if (int rc = kd::check_tree(t)) return rc;
   if (knn_variant < 0 || knn_variant > 2) return kd::set_error(KD_ERR_INVALID, "knn_variant must be 0, 1 or 2");
   if (group_size != 0 && group_size != 4 && group_size != 8 && group_size != 16 && group_size != 32 &&
@@ -405,6 +410,7 @@ int kd_tree_tune(kd_tree* t, int32_t knn_variant, int32_t group_size, int32_t ld
   t->knn_variant = knn_variant;
   t->group_size = group_size;
   t->lds_levels = lds_levels;
+  t->waves = waves;
   return KD_OK;
 }
 
@@ -509,7 +515,7 @@ int kd_best_n_within(kd_tree* t, const void* queries, uint64_t m, double radius,
 }
 
 int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double radius,
-                          uint32_t flags, int32_t* chosen3, void* stream) {
+                          uint32_t flags, int32_t* chosen4, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;
   if (k == 0 || k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_INVALID, "k out of range");
   if (!queries) return kd::set_error(KD_ERR_INVALID, "queries is null");
@@ -519,16 +525,19 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
   // candidate launch shapes: thread per query, and groups of the two smallest sizes
   // holding k, each with and without LDS-pinned stems
   int g0 = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
-  struct Cand { int variant, group, lds; };
-  std::vector<Cand> cands = {{1, 0, 0}, {1, 0, -1}, {2, g0, 0}, {2, g0, -1}};
-  if (g0 < 64) cands.push_back({2, 2 * g0, -1});
+  struct Cand { int variant, group, lds, waves; };
+  std::vector<Cand> cands = {{1, 0, 0, 0}, {1, 0, -1, 0}, {2, g0, -1, 0}};
+  if (flags & KD_SCREEN_F32)
+    for (int w = 4; w <= 6; ++w) cands.push_back({1, 0, -1, w});
+  if (g0 < 64) cands.push_back({2, 2 * g0, -1, 0});
   void* mem = nullptr;
   const size_t osz = ms * k * (sizeof(uint32_t) + sizeof(double)) + ms * sizeof(uint32_t) + 1024;
   KD_HIP(hipMalloc(&mem, osz));
   uint32_t* items = static_cast<uint32_t*>(mem);
   double* dists = reinterpret_cast<double*>(static_cast<uint8_t*>(mem) + ((ms * k * 4 + 255) / 256) * 256);
   uint32_t* counts = reinterpret_cast<uint32_t*>(dists + ms * k);
-  const int save_variant = t->knn_variant, save_group = t->group_size, save_lds = t->lds_levels;
+  const int save_variant = t->knn_variant, save_group = t->group_size, save_lds = t->lds_levels,
+            save_waves = t->waves;
   const bool save_timing = t->timing;
   int rc = kd_tree_set_timing(t, 1);
   float best_ms = 1e30f;
@@ -538,6 +547,7 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
     t->knn_variant = c.variant;
     t->group_size = c.group;
     t->lds_levels = c.lds;
+    t->waves = c.waves;
     float ms_c = 0.0f;
     for (int rep = 0; rep < 2 && rc == KD_OK; ++rep) {
       rc = kd_nearest_n(t, queries, ms, k, radius, flags, items, dists, counts, stream);
@@ -555,15 +565,18 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
     t->knn_variant = save_variant;
     t->group_size = save_group;
     t->lds_levels = save_lds;
+    t->waves = save_waves;
     return rc;
   }
   t->knn_variant = best.variant;
   t->group_size = best.group;
   t->lds_levels = best.lds;
-  if (chosen3) {
-    chosen3[0] = best.variant;
-    chosen3[1] = best.group;
-    chosen3[2] = best.lds;
+  t->waves = best.waves;
+  if (chosen4) {
+    chosen4[0] = best.variant;
+    chosen4[1] = best.group;
+    chosen4[2] = best.lds;
+    chosen4[3] = best.waves;
   }
   return KD_OK;
 }

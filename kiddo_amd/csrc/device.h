@@ -435,36 +435,45 @@ template <int MAXK> struct ScreenKnnVis {
   __device__ double bound() const { return v.bound(); }
   __device__ bool prune(double rd) const { return v.prune(rd); }
   __device__ bool unbounded() const { return v.unbounded(); }
+  // Two passes per run of up to 32 entries: the FP32 screen marks survivors in a bit mask
+  // against the threshold at the start of the run, then only survivors are measured in
+  // FP64 and offered (in entry order) to the collector. The wave then pays the FP64 and
+  // insertion work once per survivor of its busiest lane, not once per scanned entry.
   template <int K>
   __device__ void leaf(const DevTree<double, K>& t, const double (&qw)[K], uint32_t leaf) {
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
-    for (uint32_t p = s; p < e; ++p) {
-      const float4 sh = t.shadow[p];
-      float acc;
-      {
+    for (uint32_t c0 = s; c0 < e; c0 += 32) {
+      const uint32_t n = (e - c0) < 32u ? (e - c0) : 32u;
+      uint32_t mask = 0;
+      for (uint32_t j = 0; j < n; ++j) {
+        const float4 sh = t.shadow[c0 + j];
         const float dx = sub_rn(sh.x, qf[0]);
-        acc = mul_rn(dx, dx);
+        float acc = mul_rn(dx, dx);
         const float dy = sub_rn(sh.y, qf[1]);
         acc = add_rn(acc, mul_rn(dy, dy));
         if (K > 2) {
           const float dz = sub_rn(sh.z, qf[2]);
           acc = add_rn(acc, mul_rn(dz, dz));
         }
+        mask |= (acc > thr ? 0u : 1u) << j;
       }
-      if (acc > thr) continue;
-      ++rescored;
-      const double d = point_dist(t, p, qw);
-      const bool ok = v.excl ? d < v.radius : d <= v.radius;
-      if (!ok) continue;
-      const uint32_t it = __float_as_uint(sh.w);
-      if (v.cnt >= v.k && !TopK<double, MAXK>::less(d, it, v.kd, v.kit)) continue;
-      v.offer(d, it);
-      thr = screen_threshold(v.bound(), eps_k);
+      while (mask) {
+        const uint32_t j = uint32_t(__ffs(mask) - 1);
+        mask &= mask - 1u;
+        const uint32_t p = c0 + j;
+        ++rescored;
+        const double d = point_dist(t, p, qw);
+        const bool ok = v.excl ? d < v.radius : d <= v.radius;
+        if (!ok) continue;
+        if (v.cnt >= v.k && !(d <= v.kd)) continue;
+        const uint32_t it = t.items[p];
+        if (v.cnt >= v.k && !TopK<double, MAXK>::less(d, it, v.kd, v.kit)) continue;
+        v.offer(d, it);
+        thr = screen_threshold(v.bound(), eps_k);
+      }
     }
   }
 };
-
-// ---------------------------------------------------------------------- kernels --------
 
 // Sums the wave's counters and adds them to the global ones with one atomic each.
 // Every lane of the wave must reach this call.

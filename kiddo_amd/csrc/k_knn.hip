@@ -56,8 +56,8 @@ __device__ __forceinline__ void screen_setup(const double (&q)[K], const double*
   eps_k = sqrt((double)K) * eps * (1.0 + 1e-6);
 }
 
-template <int K, int MAXK>
-__global__ __launch_bounds__(kBlock) void k_nearest_n_screen(
+template <int K, int MAXK, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_nearest_n_screen(
     DevTree<double, K> t, const double* __restrict__ queries, const uint32_t* __restrict__ order,
     uint64_t m, int k, double radius, int excl, const double* __restrict__ max_abs,
     uint32_t* __restrict__ out_items, double* __restrict__ out_d, uint32_t* __restrict__ out_counts,
@@ -129,8 +129,22 @@ __global__ __launch_bounds__(kBlock) void k_knn_group(DevTree<A, K> t, const A* 
 
 namespace {
 
+template <class A, int K, int MAXK, int WAVES>
+int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts);
+
 template <class A, int K, int MAXK>
 int launch_thread(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts) {
+  // register/occupancy trade-off of the screened kernel (chosen by the autotuner)
+  switch (a.t->waves) {
+    case 4: return launch_thread_w<A, K, MAXK, 4>(a, k, radius, items, dists, counts);
+    case 5: return launch_thread_w<A, K, MAXK, 5>(a, k, radius, items, dists, counts);
+    case 6: return launch_thread_w<A, K, MAXK, 6>(a, k, radius, items, dists, counts);
+    default: return launch_thread_w<A, K, MAXK, 3>(a, k, radius, items, dists, counts);
+  }
+}
+
+template <class A, int K, int MAXK, int WAVES>
+int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts) {
   using O = typename dev::Ax<A>::O;
   kd_tree* t = a.t;
   const int excl = (a.flags & KD_EXCLUSIVE) ? 1 : 0;
@@ -141,7 +155,7 @@ int launch_thread(const QueryArgs& a, int k, double radius, uint32_t* items, voi
   KernelTimer kt(t, a.s);
   if constexpr (std::is_same<A, double>::value && K <= 3) {
     if (a.flags & KD_SCREEN_F32) {
-      hipLaunchKernelGGL((dev::k_nearest_n_screen<K, MAXK>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds, a.s,
+      hipLaunchKernelGGL((dev::k_nearest_n_screen<K, MAXK, WAVES>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds, a.s,
                          dt, q, a.order, a.m, k, radius, excl, t->d_max_abs, items,
                          static_cast<double*>(dists), counts, t->counters);
       KD_HIP(hipGetLastError());

@@ -36,6 +36,7 @@ struct kd_tree {
   int knn_variant = 0;  // 0 auto, 1 thread per query, 2 group per query
   int group_size = 0;   // 0 auto (smallest of 4/8/16/32/64 holding k)
   int lds_levels = -1;  // -1 auto
+  int waves = 0;        // minimum waves per SIMD the k-NN kernels are compiled for (0 = 3)
   // optional HIP-event timing of the main query kernel of each call
   bool timing = false;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;

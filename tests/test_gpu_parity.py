@@ -178,13 +178,13 @@ def test_screen_f32_equals_plain_f64(ka, oracle_mod, cuda):
     ot = O.build(pts, 32)
     for k in (1, 8, 16):
         ref = O.nearest_n(ot, q, k)
-        for variant in (1, 2):
-            shadow.tune(variant, 0, -1)
+        for variant, waves in ((1, 0), (2, 0), (1, 4), (1, 6)):
+            shadow.tune(variant, 0, -1, waves)
             for flags in (ka.KD_SCREEN_F32, ka.KD_SCREEN_F32 | ka.KD_MORTON):
                 r = shadow.nearest_n_raw(q, k, flags=flags)
                 got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
                 for a, b in zip(got, ref):
-                    assert_same_bits(a, b, f"screen k={k} flags={flags} variant={variant}")
+                    assert_same_bits(a, b, f"screen k={k} flags={flags} variant={variant}/{waves}")
         screened, rescored = shadow.screen_stats()
         assert 0 < rescored < screened
         r = plain.nearest_n_raw(q, k)
