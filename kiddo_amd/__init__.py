@@ -158,8 +158,11 @@ class HostLayout:
         _check(lib().kd_layout_desc(self._h, C.byref(self.desc)), "kd_layout_desc")
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().kd_layout_free(self._h)
+        if getattr(self, "_h", None) and _lib is not None:
+            try:
+                _lib.kd_layout_free(self._h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     @property
