@@ -49,7 +49,8 @@ enum kd_query_flags {
   KD_EXCLUSIVE = 1u << 0, /* `.exclusive_boundaries()`: strict `< radius`              */
   KD_UNSORTED = 1u << 1,  /* `.unsorted()`: within results in visit order              */
   KD_MORTON = 1u << 2,    /* traverse queries in Morton order (results keep input order) */
-  KD_SCREEN_F32 = 1u << 3 /* f64 trees: FP32 screen + FP64 rescore (same results)      */
+  KD_SCREEN_F32 = 1u << 3, /* f64 trees: FP32 screen + FP64 rescore (same results)     */
+  KD_GRAPH = 1u << 4        /* forests: capture the launch in a hipGraph and replay it     */
 };
 
 /* tree-create flags */
@@ -175,6 +176,36 @@ int kd_query_stats(kd_tree* tree, uint64_t* out4);
  * last timed call and returns its kernel's elapsed milliseconds. */
 int kd_tree_set_timing(kd_tree* tree, int32_t enable);
 int kd_last_kernel_ms(kd_tree* tree, float* ms);
+
+/* ------------------------------------------------------------- batched forests ---- */
+/* Thousands of independent trees of uneven sizes, each with its own query set, answered by
+ * one persistent launch fed by a device work queue (no Kiddo counterpart: the reference
+ * would loop over trees, each query through KdTree::query, builder.rs:2318-2331).
+ * Trees are given in Kiddo layout (kd_tree_desc, same dtype and dims for all). */
+typedef struct kd_forest kd_forest;
+int kd_forest_create(const kd_tree_desc* descs, uint32_t ntrees, uint32_t tree_flags, kd_forest** out);
+int kd_forest_destroy(kd_forest* forest);
+/* Plans a batch: query_offsets (host, ntrees+1) delimit each tree's queries in one
+ * concatenated device array; work items of up to `chunk` queries (0 = 256) are ordered by
+ * cost; KD_MORTON also Morton-orders the queries inside each tree's range (synchronous). */
+int kd_forest_plan(kd_forest* forest, const void* queries, const uint64_t* query_offsets,
+                   uint32_t chunk, uint32_t flags, void* stream);
+/* nearest_n over the planned batch; outputs indexed by position in the concatenated
+ * queries. KD_GRAPH captures the launch once and replays it (same pointers/arguments). */
+int kd_forest_nearest_n(kd_forest* forest, const void* queries, uint32_t k, double radius,
+                        uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream);
+/* within over the planned batch: first with items == NULL (writes offsets[m+1]), then with
+ * output buffers of offsets[m] entries. */
+int kd_forest_within(kd_forest* forest, const void* queries, double radius, uint32_t flags,
+                     uint64_t* offsets, uint32_t* items, void* dists, void* stream);
+int kd_forest_tune(kd_forest* forest, int32_t variant, int32_t group_size, int32_t lds_levels,
+                   int32_t waves);
+int kd_forest_autotune_nearest_n(kd_forest* forest, const void* queries, uint32_t k, double radius,
+                                 uint32_t flags, uint32_t* items, void* dists, uint32_t* counts,
+                                 int32_t* chosen4, void* stream);
+int kd_forest_set_timing(kd_forest* forest, int32_t enable);
+int kd_forest_last_kernel_ms(kd_forest* forest, float* ms);
+int kd_forest_query_stats(kd_forest* forest, uint64_t* out4);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,7 @@ KD_EXCLUSIVE = 1 << 0
 KD_UNSORTED = 1 << 1
 KD_MORTON = 1 << 2
 KD_SCREEN_F32 = 1 << 3
+KD_GRAPH = 1 << 4
 KD_TREE_SHADOW_F32 = 1 << 0
 
 _NP_CODE = {np.dtype(np.float32): KD_F32, np.dtype(np.float64): KD_F64,
@@ -84,6 +85,16 @@ SIGNATURES = {
     "kd_within_fill": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_best_n_within": ([_vp, _vp, _u64, _f64, _u32, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_query_stats": ([_vp, _vp], C.c_int),
+    "kd_forest_create": ([_vp, _u32, _u32, C.POINTER(_vp)], C.c_int),
+    "kd_forest_destroy": ([_vp], C.c_int),
+    "kd_forest_plan": ([_vp, _vp, _vp, _u32, _u32, _vp], C.c_int),
+    "kd_forest_nearest_n": ([_vp, _vp, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_forest_within": ([_vp, _vp, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_forest_tune": ([_vp, _i32, _i32, _i32, _i32], C.c_int),
+    "kd_forest_autotune_nearest_n": ([_vp, _vp, _u32, _f64, _u32, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_forest_set_timing": ([_vp, _i32], C.c_int),
+    "kd_forest_last_kernel_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
+    "kd_forest_query_stats": ([_vp, _vp], C.c_int),
     "kd_tree_set_timing": ([_vp, _i32], C.c_int),
     "kd_last_kernel_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
 }
@@ -391,6 +402,103 @@ class ImmutableKdTree:
         return out
 
 
+class Forest:
+    """Many independent trees (same dtype and K) in pooled HBM buffers, queried as one batch
+    by a persistent kernel fed from a device work queue (kd_forest_* in kiddo_amd.h)."""
+
+    def __init__(self, layouts):
+        """layouts: HostLayout objects (or anything with a `.desc` TreeDesc)."""
+        self._layouts = list(layouts)
+        arr = (TreeDesc * len(self._layouts))(*[l.desc for l in self._layouts])
+        h = C.c_void_p()
+        _check(lib().kd_forest_create(arr, len(self._layouts), 0, C.byref(h)), "kd_forest_create")
+        self._h = h
+        self.ntrees = len(self._layouts)
+        self.dtype_code = self._layouts[0].dtype_code
+        self.K = self._layouts[0].K
+        self.m = 0
+        self._layouts = None  # the device pools hold everything now
+
+    @classmethod
+    def from_points(cls, point_sets, bucket=32, threads=0):
+        return cls([HostLayout(p, bucket, None, threads) for p in point_sets])
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kd_forest_destroy(self._h)
+            self._h = None
+
+    @property
+    def out_dtype(self):
+        return _O_NP[self.dtype_code]
+
+    def plan(self, queries, query_offsets, chunk=256, morton=True, stream=None):
+        """queries: concatenated (m, K) device tensor; query_offsets: ntrees+1 host ints."""
+        offs = np.ascontiguousarray(query_offsets, dtype=np.uint64)
+        assert offs.size == self.ntrees + 1
+        self.m = int(offs[-1])
+        _check(lib().kd_forest_plan(self._h, queries.data_ptr(), _np_ptr(offs), chunk,
+                                    KD_MORTON if morton else 0, _stream_ptr(stream)),
+               "kd_forest_plan")
+        self.query_offsets = offs
+
+    def tune(self, variant=1, group_size=0, lds_levels=0, waves=0):
+        _check(lib().kd_forest_tune(self._h, variant, group_size, lds_levels, waves), "kd_forest_tune")
+
+    def _knn_out(self, k):
+        torch = _torch()
+        dt = {np.float32: torch.float32, np.float64: torch.float64}[self.out_dtype]
+        return NearestN(torch.empty((self.m, k), dtype=torch.uint32, device="cuda"),
+                        torch.empty((self.m, k), dtype=dt, device="cuda"),
+                        torch.empty(self.m, dtype=torch.uint32, device="cuda"))
+
+    def nearest_n(self, queries, k, radius=math.inf, flags=0, out=None, stream=None):
+        out = out or self._knn_out(k)
+        _check(lib().kd_forest_nearest_n(self._h, queries.data_ptr(), k, float(radius), flags,
+                                         out.items.data_ptr(), out.distances.data_ptr(),
+                                         out.counts.data_ptr(), _stream_ptr(stream)),
+               "kd_forest_nearest_n")
+        return out
+
+    def autotune_nearest_n(self, queries, k, radius=math.inf, flags=0, stream=None):
+        out = self._knn_out(k)
+        chosen = np.zeros(4, np.int32)
+        _check(lib().kd_forest_autotune_nearest_n(self._h, queries.data_ptr(), k, float(radius),
+                                                  flags, out.items.data_ptr(),
+                                                  out.distances.data_ptr(), out.counts.data_ptr(),
+                                                  _np_ptr(chosen), _stream_ptr(stream)),
+               "kd_forest_autotune_nearest_n")
+        return dict(zip(("variant", "group_size", "lds_levels", "waves"), (int(x) for x in chosen)))
+
+    def within(self, queries, radius, flags=0, stream=None):
+        torch = _torch()
+        offsets = torch.empty(self.m + 1, dtype=torch.int64, device="cuda")
+        sp = _stream_ptr(stream)
+        _check(lib().kd_forest_within(self._h, queries.data_ptr(), float(radius), flags,
+                                      offsets.data_ptr(), None, None, sp), "kd_forest_within(count)")
+        total = int(offsets[self.m].item()) if self.m else 0
+        dt = {np.float32: torch.float32, np.float64: torch.float64}[self.out_dtype]
+        items = torch.empty(max(total, 1), dtype=torch.uint32, device="cuda")
+        d = torch.empty(max(total, 1), dtype=dt, device="cuda")
+        _check(lib().kd_forest_within(self._h, queries.data_ptr(), float(radius), flags,
+                                      offsets.data_ptr(), items.data_ptr(), d.data_ptr(), sp),
+               "kd_forest_within(fill)")
+        return Within(offsets, items[:total], d[:total])
+
+    def set_timing(self, enable=True):
+        _check(lib().kd_forest_set_timing(self._h, int(enable)), "kd_forest_set_timing")
+
+    def last_kernel_ms(self):
+        ms = C.c_float()
+        _check(lib().kd_forest_last_kernel_ms(self._h, C.byref(ms)), "kd_forest_last_kernel_ms")
+        return ms.value
+
+    def query_stats(self):
+        out = np.zeros(4, np.uint64)
+        _check(lib().kd_forest_query_stats(self._h, _np_ptr(out)), "kd_forest_query_stats")
+        return dict(zip(("leaves", "points", "stems", "rescored"), (int(x) for x in out)))
+
+
 class QueryBuilder:
     """Fluent batch builder mirroring kiddo src/kd_tree/query/builder.rs."""
 
@@ -475,6 +583,6 @@ class QueryBuilder:
         raise ValueError("choose a query family first")
 
 
-__all__ = ["ImmutableKdTree", "HostLayout", "QueryBuilder", "NearestOne", "NearestN",
+__all__ = ["ImmutableKdTree", "HostLayout", "Forest", "KD_GRAPH", "QueryBuilder", "NearestOne", "NearestN",
            "Within", "KiddoAmdError", "lib", "SIGNATURES", "KD_EXCLUSIVE", "KD_UNSORTED",
            "KD_MORTON", "KD_SCREEN_F32", "KD_TREE_SHADOW_F32"]
