@@ -57,8 +57,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
     ap.add_argument("--no-latency", action="store_true")
-    ap.add_argument("--workload", choices=["knn16", "forest"], default="knn16",
-                    help="knn16: the headline 16M-point k-NN; forest: 4096-tree batched form")
+    ap.add_argument("--workload", choices=["knn16", "forest", "serving"], default="knn16",
+                    help="knn16: the headline 16M-point k-NN; forest: 4096-tree batched form; "
+                         "serving: continuous batching over resident trees")
+    ap.add_argument("--requests", type=int, default=3000)
+    ap.add_argument("--rate", type=float, default=4000.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--trees", type=int, default=4096)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
@@ -79,6 +82,8 @@ def main():
 
     if args.workload == "forest":
         return bench_forest(args, torch, ka)
+    if args.workload == "serving":
+        return bench_serving(args, torch, ka)
 
     if args.copy_probe:
         # A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it scales the
@@ -342,6 +347,94 @@ def bench_forest(args, torch, ka):
                      "unit": "GB/s", "frac": round(algo / kern_s / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_forest_knn", "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
+    }), flush=True)
+
+
+def bench_serving(args, torch, ka):
+    """Serving form: a resident f64 3-D tree of 12,000,000 unit vectors (70% from 200
+    Gaussian lat/lon clusters, sigma 0.5 deg, 30% uniform on the sphere; items are the rank of
+    a seeded per-item f32 weight) and a u16 4-D palette tree of 65,536 colours. Requests
+    arrive as a Poisson process; sizes are log-uniform in [1, 65536]; each is nearest_one,
+    nearest_n(5) or best_n_within(r, 5) (r = 0.001 on the sphere, 2000 per channel on the
+    palette), on either tree. Reports whole-run queries/s and per-request p50/p99 latency."""
+    import threading
+    from kiddo_amd.datasets import latlon_clusters, palette_u16, weight_rank_items
+
+    t0 = time.time()
+    pts = latlon_clusters(12_000_000, seed=args.seed)
+    items = weight_rank_items(len(pts), seed=args.seed + 1)
+    pal = palette_u16(65536, seed=args.seed + 2)
+    sphere = ka.ImmutableKdTree.new_from_entries(pts, items, args.bucket)
+    palette = ka.ImmutableKdTree.new_from_slice(pal, args.bucket)
+    log(f"serving trees built and resident in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(args.seed + 3)
+    fams = [ka.FAMILY_NEAREST_ONE, ka.FAMILY_NEAREST_N, ka.FAMILY_BEST_N_WITHIN]
+    reqs = []
+    for i in range(args.requests):
+        tree = int(rng.random() < 0.25)
+        fam = fams[int(rng.integers(0, 3))]
+        m = int(np.exp(rng.uniform(0, np.log(65536))))
+        if tree == 0:
+            q = latlon_clusters(m, seed=100_000 + i)
+            radius = 0.001 ** 2
+        else:
+            q = palette_u16(m, seed=200_000 + i)
+            radius = float(4 * 2000 ** 2)
+        flags = ka.KD_MORTON if m >= 4096 else 0
+        reqs.append((tree, fam, q, 5, radius if fam == ka.FAMILY_BEST_N_WITHIN else np.inf, flags))
+    gaps = rng.exponential(1.0 / args.rate, args.requests)
+    arrivals = np.cumsum(gaps)
+    total_q = sum(r[2].shape[0] for r in reqs)
+    log(f"{args.requests} requests, {total_q} queries, offered {total_q / arrivals[-1] / 1e6:.1f} Mq/s")
+    srv = ka.Server([sphere, palette], arena_bytes=1 << 30, slots=4, max_coalesce=1 << 18)
+    # warm-up: one request of each family and tree
+    for tree in (0, 1):
+        for fam in fams:
+            r = reqs[0]
+            q = latlon_clusters(256, seed=7) if tree == 0 else palette_u16(256, seed=7)
+            srv.wait(srv.submit(tree, fam, q, 5, r[4]))
+    tickets = [None] * len(reqs)
+    lat = [None] * len(reqs)
+    start = time.perf_counter()
+
+    def producer():
+        for i, (tree, fam, q, k, radius, flags) in enumerate(reqs):
+            while time.perf_counter() - start < arrivals[i]:
+                time.sleep(0.0001)
+            tickets[i] = srv.submit(tree, fam, q, k, radius, flags)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    for i in range(len(reqs)):
+        while tickets[i] is None:
+            time.sleep(0.0001)
+        lat[i] = srv.wait(tickets[i])[3]
+    end = time.perf_counter()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    by_fam = {}
+    names = {ka.FAMILY_NEAREST_ONE: "nearest_one", ka.FAMILY_NEAREST_N: "nearest_n5",
+             ka.FAMILY_BEST_N_WITHIN: "best_n_within5"}
+    for fam in fams:
+        sel = np.array([r[1] == fam for r in reqs])
+        by_fam[names[fam]] = {"requests": int(sel.sum()), "p50_ms": round(float(np.percentile(lat[sel], 50)), 3),
+                              "p99_ms": round(float(np.percentile(lat[sel], 99)), 3)}
+    value = total_q / (end - start)
+    log(f"serving: {value / 1e6:.1f} Mq/s, p50 {np.percentile(lat, 50):.2f} ms, p99 {np.percentile(lat, 99):.2f} ms, {st}")
+    print(json.dumps({
+        "metric": "queries/s and p50/p99 request latency, continuous batching over resident trees",
+        "value": round(value, 1), "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round((end - start) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 + u16(f64 output)", "data": "synthetic (seeded)",
+        "config": {"workload": "serving-12M-f64-sphere+u16-palette", "requests": args.requests,
+                   "rate_per_s": args.rate, "queries": int(total_q), "slots": 4,
+                   "arena_bytes": 1 << 30},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "per_family": by_fam, "server": st,
+        "offered_qps": round(total_q / arrivals[-1], 1),
     }), flush=True)
 
 

@@ -207,6 +207,45 @@ int kd_forest_set_timing(kd_forest* forest, int32_t enable);
 int kd_forest_last_kernel_ms(kd_forest* forest, float* ms);
 int kd_forest_query_stats(kd_forest* forest, uint64_t* out4);
 
+/* ------------------------------------------------------------------- serving ---- */
+/* A continuous-batching request loop over resident trees (no Kiddo counterpart; it serves
+ * the same families as KdTree::query). Requests carry HOST query arrays and HOST result
+ * buffers and may be submitted from any thread. The loop admits requests in arrival order
+ * when one of `slots` streams is free and the device arena (`arena_bytes`, mirrored in
+ * pinned host memory) has room for the request's queries and results; it coalesces later
+ * pending requests of the same kind (tree, family, k, radius, flags) into the same launch,
+ * up to `max_coalesce` queries; launches on different slots overlap on the device. */
+enum kd_family {
+  KD_FAMILY_NEAREST_ONE = 1,   /* items[m], dists[m]                                   */
+  KD_FAMILY_NEAREST_N = 2,     /* items[m*k], dists[m*k], counts[m]; radius as kd_nearest_n */
+  KD_FAMILY_BEST_N_WITHIN = 3, /* items[m*k], dists[m*k], counts[m]; k is n             */
+  KD_FAMILY_APPROX = 4         /* items[m], dists[m]                                   */
+};
+typedef struct kd_request {
+  int32_t tree;       /* index into the server's trees */
+  int32_t family;     /* kd_family */
+  uint32_t k;
+  uint32_t flags;     /* KD_EXCLUSIVE, KD_MORTON, KD_SCREEN_F32 */
+  double radius;      /* +INFINITY for none */
+  const void* queries; /* host, m x dims of the tree's dtype; read during kd_server_submit's admission */
+  uint64_t m;
+  uint32_t* items;    /* host outputs, valid once kd_server_wait returns */
+  void* dists;
+  uint32_t* counts;
+} kd_request;
+typedef struct kd_server kd_server;
+int kd_server_create(kd_tree* const* trees, uint32_t ntrees, uint64_t arena_bytes, uint32_t slots,
+                     uint64_t max_coalesce, kd_server** out);
+/* Non-blocking; the request's buffers must stay valid until kd_server_wait returns. */
+int kd_server_submit(kd_server* server, const kd_request* request, uint64_t* ticket);
+/* Blocks until the request is answered; returns its status and its latency (submit to
+ * results in host memory). Each ticket is waited for once. */
+int kd_server_wait(kd_server* server, uint64_t ticket, double* latency_ms);
+/* out6 = {requests done, launches, queries launched, requests coalesced into another's
+ * launch, admissions deferred for memory, peak arena bytes in use}. */
+int kd_server_stats(kd_server* server, uint64_t* out6);
+int kd_server_destroy(kd_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,11 @@ SIGNATURES = {
     "kd_forest_set_timing": ([_vp, _i32], C.c_int),
     "kd_forest_last_kernel_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
     "kd_forest_query_stats": ([_vp, _vp], C.c_int),
+    "kd_server_create": ([_vp, _u32, _u64, _u32, _u64, C.POINTER(_vp)], C.c_int),
+    "kd_server_submit": ([_vp, _vp, C.POINTER(_u64)], C.c_int),
+    "kd_server_wait": ([_vp, _u64, C.POINTER(C.c_double)], C.c_int),
+    "kd_server_stats": ([_vp, _vp], C.c_int),
+    "kd_server_destroy": ([_vp], C.c_int),
     "kd_tree_set_timing": ([_vp, _i32], C.c_int),
     "kd_last_kernel_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
 }
@@ -499,6 +504,70 @@ class Forest:
         return dict(zip(("leaves", "points", "stems", "rescored"), (int(x) for x in out)))
 
 
+class Request(C.Structure):
+    _fields_ = [("tree", C.c_int32), ("family", C.c_int32), ("k", C.c_uint32),
+                ("flags", C.c_uint32), ("radius", C.c_double), ("queries", C.c_void_p),
+                ("m", C.c_uint64), ("items", C.c_void_p), ("dists", C.c_void_p),
+                ("counts", C.c_void_p)]
+
+
+FAMILY_NEAREST_ONE, FAMILY_NEAREST_N, FAMILY_BEST_N_WITHIN, FAMILY_APPROX = 1, 2, 3, 4
+
+
+class Server:
+    """Continuous-batching request loop over resident trees (kd_server_* in kiddo_amd.h).
+    Requests take host numpy queries; results come back as host numpy arrays."""
+
+    def __init__(self, trees, arena_bytes=256 << 20, slots=4, max_coalesce=1 << 18):
+        self.trees = list(trees)
+        arr = (C.c_void_p * len(self.trees))(*[t.handle.value for t in self.trees])
+        h = C.c_void_p()
+        _check(lib().kd_server_create(arr, len(self.trees), arena_bytes, slots, max_coalesce,
+                                      C.byref(h)), "kd_server_create")
+        self._h = h
+        self._live = {}
+
+    def submit(self, tree, family, queries, k=1, radius=math.inf, flags=0):
+        t = self.trees[tree]
+        q = np.ascontiguousarray(queries, dtype=t.a_dtype)
+        m = q.shape[0]
+        w = 1 if family in (FAMILY_NEAREST_ONE, FAMILY_APPROX) else k
+        out_dt = t.a_dtype if family == FAMILY_APPROX else t.out_dtype
+        items = np.empty((m, w), np.uint32)
+        dists = np.empty((m, w), out_dt)
+        counts = np.empty(m, np.uint32)
+        r = Request(tree, family, k, flags, float(radius), q.ctypes.data, m, items.ctypes.data,
+                    dists.ctypes.data, counts.ctypes.data)
+        ticket = C.c_uint64()
+        _check(lib().kd_server_submit(self._h, C.byref(r), C.byref(ticket)), "kd_server_submit")
+        self._live[ticket.value] = (q, items, dists, counts, family)
+        return ticket.value
+
+    def wait(self, ticket):
+        """Returns (items, dists, counts, latency_ms); nearest_one rows are 1 wide."""
+        lat = C.c_double()
+        rc = lib().kd_server_wait(self._h, ticket, C.byref(lat))
+        q, items, dists, counts, family = self._live.pop(ticket)
+        _check(rc, "kd_server_wait")
+        if family in (FAMILY_NEAREST_ONE, FAMILY_APPROX):
+            return items[:, 0], dists[:, 0], None, lat.value
+        return items, dists, counts, lat.value
+
+    def stats(self):
+        out = np.zeros(6, np.uint64)
+        _check(lib().kd_server_stats(self._h, _np_ptr(out)), "kd_server_stats")
+        return dict(zip(("done", "launches", "queries", "coalesced", "memory_waits",
+                         "peak_arena_bytes"), (int(x) for x in out)))
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 class QueryBuilder:
     """Fluent batch builder mirroring kiddo src/kd_tree/query/builder.rs."""
 
@@ -583,6 +652,7 @@ class QueryBuilder:
         raise ValueError("choose a query family first")
 
 
-__all__ = ["ImmutableKdTree", "HostLayout", "Forest", "KD_GRAPH", "QueryBuilder", "NearestOne", "NearestN",
+__all__ = ["ImmutableKdTree", "HostLayout", "Forest", "Server", "KD_GRAPH", "FAMILY_NEAREST_ONE",
+           "FAMILY_NEAREST_N", "FAMILY_BEST_N_WITHIN", "FAMILY_APPROX", "QueryBuilder", "NearestOne", "NearestN",
            "Within", "KiddoAmdError", "lib", "SIGNATURES", "KD_EXCLUSIVE", "KD_UNSORTED",
            "KD_MORTON", "KD_SCREEN_F32", "KD_TREE_SHADOW_F32"]

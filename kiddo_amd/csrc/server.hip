@@ -1,0 +1,411 @@
+// server.hip — continuous-batching request loop over resident trees.
+//
+// Requests (host query arrays and host result buffers) are submitted from any thread. A
+// single loop thread
+//   * admits pending requests in arrival order whenever a stream slot is free and the device
+//     result arena has room for the request's queries and results (admission by memory);
+//   * coalesces the admitted request with later pending requests of the same kind (same tree,
+//     family, k, radius and flags) into one launch, up to `max_coalesce` queries;
+//   * runs each launch on its slot's stream: host -> pinned staging -> HBM, the query kernel
+//     through the C ABI, HBM -> pinned staging, with an event at the end;
+//   * retires finished launches by polling their events, copies every request's slice of the
+//     results into its host buffers, frees the arena range and the slot, and stamps latency.
+// Launches on different slots overlap on the device; a request's results do not depend on
+// what it was batched with, because every query is answered independently.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "tree.h"
+
+namespace kd {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+  uint64_t ticket;
+  kd_request r;
+  Clock::time_point submitted;
+  double latency_ms = 0;
+  bool done = false;
+  int status = KD_OK;
+};
+
+// First-fit allocator over one device arena (and its pinned mirror); host bookkeeping only.
+struct Arena {
+  uint64_t size = 0;
+  std::map<uint64_t, uint64_t> free_;  // offset -> length
+  void init(uint64_t n) {
+    size = n;
+    free_.clear();
+    free_[0] = n;
+  }
+  bool alloc(uint64_t n, uint64_t& off) {
+    n = (n + 255) & ~uint64_t(255);
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= n) {
+        off = it->first;
+        const uint64_t rest = it->second - n;
+        free_.erase(it);
+        if (rest) free_[off + n] = rest;
+        return true;
+      }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    n = (n + 255) & ~uint64_t(255);
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  uint64_t in_use() const {
+    uint64_t f = 0;
+    for (auto& kv : free_) f += kv.second;
+    return size - f;
+  }
+};
+
+struct Launch {
+  int slot;
+  std::vector<Req*> reqs;
+  std::vector<uint64_t> q_begin;  // per request: first query index in the launch
+  uint64_t m = 0, off = 0, bytes = 0;
+  uint64_t q_bytes = 0, items_off = 0, dists_off = 0, counts_off = 0;
+  uint32_t k = 0;  // result row width (1 for nearest_one)
+  size_t o_size = 0;
+  hipEvent_t done_ev = nullptr;
+};
+
+}  // namespace
+}  // namespace kd
+
+struct kd_server {
+  std::vector<kd_tree*> trees;
+  uint32_t slots = 0;
+  uint64_t max_coalesce = 0;
+  uint8_t* dev = nullptr;
+  uint8_t* pinned = nullptr;
+  kd::Arena arena;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;
+  std::vector<bool> slot_busy;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<kd::Req*> submitted;
+  std::map<uint64_t, kd::Req*> reqs;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  std::thread loop;
+  // statistics
+  std::atomic<uint64_t> st_done{0}, st_launches{0}, st_queries{0}, st_coalesced{0}, st_mem_waits{0};
+  uint64_t peak_in_use = 0;
+  int device = 0;
+};
+
+namespace kd {
+namespace {
+
+size_t out_size(const kd_tree* t) { return (t->dtype == KD_F32) ? 4 : 8; }
+size_t a_size(const kd_tree* t) { return dtype_size(t->dtype); }
+
+bool same_kind(const kd_request& a, const kd_request& b) {
+  return a.tree == b.tree && a.family == b.family && a.k == b.k && a.flags == b.flags &&
+         std::memcmp(&a.radius, &b.radius, sizeof(double)) == 0;
+}
+
+uint64_t launch_bytes(const kd_tree* t, const kd_request& r, uint64_t m) {
+  const uint64_t w = r.family == KD_FAMILY_NEAREST_ONE || r.family == KD_FAMILY_APPROX ? 1 : r.k;
+  auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+  return al(m * t->dims * a_size(t)) + al(m * w * 4) + al(m * w * out_size(t)) + al(m * 4);
+}
+
+int run_launch(kd_server* S, Launch& L) {
+  kd_tree* t = S->trees[L.reqs[0]->r.tree];
+  const kd_request& r0 = L.reqs[0]->r;
+  hipStream_t s = S->streams[L.slot];
+  auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+  L.k = (r0.family == KD_FAMILY_NEAREST_ONE || r0.family == KD_FAMILY_APPROX) ? 1 : r0.k;
+  L.o_size = out_size(t);
+  L.q_bytes = L.m * t->dims * a_size(t);
+  L.items_off = L.off + al(L.q_bytes);
+  L.dists_off = L.items_off + al(L.m * L.k * 4);
+  L.counts_off = L.dists_off + al(L.m * L.k * L.o_size);
+  // gather the requests' queries into pinned staging, then one copy to HBM
+  const size_t row = t->dims * a_size(t);
+  for (size_t i = 0; i < L.reqs.size(); ++i)
+    std::memcpy(S->pinned + L.off + L.q_begin[i] * row, L.reqs[i]->r.queries, L.reqs[i]->r.m * row);
+  KD_HIP(hipMemcpyAsync(S->dev + L.off, S->pinned + L.off, L.q_bytes, hipMemcpyHostToDevice, s));
+  void* q = S->dev + L.off;
+  uint32_t* items = reinterpret_cast<uint32_t*>(S->dev + L.items_off);
+  void* dists = S->dev + L.dists_off;
+  uint32_t* counts = reinterpret_cast<uint32_t*>(S->dev + L.counts_off);
+  int rc;
+  switch (r0.family) {
+    case KD_FAMILY_NEAREST_ONE: rc = kd_nearest_one(t, q, L.m, r0.flags, items, dists, s); break;
+    case KD_FAMILY_APPROX: rc = kd_approx_nearest_one(t, q, L.m, r0.flags, items, dists, s); break;
+    case KD_FAMILY_NEAREST_N: rc = kd_nearest_n(t, q, L.m, r0.k, r0.radius, r0.flags, items, dists, counts, s); break;
+    case KD_FAMILY_BEST_N_WITHIN:
+      rc = kd_best_n_within(t, q, L.m, r0.radius, r0.k, r0.flags, items, dists, counts, s);
+      break;
+    default: rc = set_error(KD_ERR_UNSUPPORTED, "family %d is not served", r0.family);
+  }
+  if (rc) return rc;
+  const uint64_t out_bytes = L.counts_off + L.m * 4 - L.items_off;
+  KD_HIP(hipMemcpyAsync(S->pinned + L.items_off, S->dev + L.items_off, out_bytes, hipMemcpyDeviceToHost, s));
+  KD_HIP(hipEventRecord(S->events[L.slot], s));
+  L.done_ev = S->events[L.slot];
+  return KD_OK;
+}
+
+void finish_launch(kd_server* S, Launch& L, int status) {
+  const auto now = Clock::now();
+  for (size_t i = 0; i < L.reqs.size(); ++i) {
+    Req* R = L.reqs[i];
+    const uint64_t b = L.q_begin[i], m = R->r.m;
+    if (status == KD_OK) {
+      if (R->r.items) std::memcpy(R->r.items, S->pinned + L.items_off + b * L.k * 4, m * L.k * 4);
+      if (R->r.dists)
+        std::memcpy(R->r.dists, S->pinned + L.dists_off + b * L.k * L.o_size, m * L.k * L.o_size);
+      if (R->r.counts && L.k > 1 && R->r.family != KD_FAMILY_NEAREST_ONE)
+        std::memcpy(R->r.counts, S->pinned + L.counts_off + b * 4, m * 4);
+    }
+    R->status = status;
+    R->latency_ms = std::chrono::duration<double, std::milli>(now - R->submitted).count();
+  }
+  std::lock_guard<std::mutex> g(S->mu);
+  S->arena.release(L.off, L.bytes);
+  S->slot_busy[L.slot] = false;
+  for (Req* R : L.reqs) R->done = true;
+  S->st_done += L.reqs.size();
+  S->cv_done.notify_all();
+}
+
+void serve(kd_server* S) {
+  (void)hipSetDevice(S->device);
+  std::vector<Launch> inflight;
+  std::deque<Req*> pending;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> g(S->mu);
+      if (pending.empty() && inflight.empty() && S->submitted.empty())
+        S->cv_work.wait_for(g, std::chrono::microseconds(200), [&] { return S->stop || !S->submitted.empty(); });
+      if (S->stop && pending.empty() && inflight.empty() && S->submitted.empty()) return;
+      while (!S->submitted.empty()) {
+        pending.push_back(S->submitted.front());
+        S->submitted.pop_front();
+      }
+    }
+    // retire finished launches
+    for (size_t i = 0; i < inflight.size();) {
+      if (hipEventQuery(inflight[i].done_ev) == hipSuccess) {
+        finish_launch(S, inflight[i], KD_OK);
+        inflight.erase(inflight.begin() + i);
+      } else {
+        ++i;
+      }
+    }
+    // admit in arrival order while a slot and memory are free
+    bool progressed = false;
+    while (!pending.empty()) {
+      int slot = -1;
+      {
+        std::lock_guard<std::mutex> g(S->mu);
+        for (uint32_t j = 0; j < S->slots; ++j)
+          if (!S->slot_busy[j]) {
+            slot = int(j);
+            break;
+          }
+      }
+      if (slot < 0) break;
+      Req* head = pending.front();
+      kd_tree* t = S->trees[head->r.tree];
+      Launch L;
+      L.slot = slot;
+      L.reqs.push_back(head);
+      L.q_begin.push_back(0);
+      L.m = head->r.m;
+      for (auto it = std::next(pending.begin()); it != pending.end(); ++it) {
+        if (!same_kind((*it)->r, head->r)) continue;
+        if (L.m + (*it)->r.m > S->max_coalesce) break;
+        L.q_begin.push_back(L.m);
+        L.reqs.push_back(*it);
+        L.m += (*it)->r.m;
+      }
+      // shrink the group until it fits the arena; the head alone must eventually fit
+      bool ok = false;
+      {
+        std::lock_guard<std::mutex> g(S->mu);
+        while (!L.reqs.empty()) {
+          L.bytes = launch_bytes(t, head->r, L.m);
+          if (S->arena.alloc(L.bytes, L.off)) {
+            ok = true;
+            break;
+          }
+          if (L.reqs.size() == 1) break;
+          L.m -= L.reqs.back()->r.m;
+          L.reqs.pop_back();
+          L.q_begin.pop_back();
+        }
+        if (ok) {
+          S->slot_busy[slot] = true;
+          S->peak_in_use = std::max(S->peak_in_use, S->arena.in_use());
+        }
+      }
+      if (!ok) {
+        ++S->st_mem_waits;
+        break;  // wait for memory to free up (head of line keeps arrival order)
+      }
+      for (Req* R : L.reqs) pending.erase(std::find(pending.begin(), pending.end(), R));
+      ++S->st_launches;
+      S->st_queries += L.m;
+      S->st_coalesced += L.reqs.size() - 1;
+      const int rc = run_launch(S, L);
+      if (rc) {
+        finish_launch(S, L, rc);
+      } else {
+        inflight.push_back(L);
+      }
+      progressed = true;
+    }
+    if (!progressed && !inflight.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+}  // namespace
+}  // namespace kd
+
+extern "C" {
+
+int kd_server_create(kd_tree* const* trees, uint32_t ntrees, uint64_t arena_bytes, uint32_t slots,
+                     uint64_t max_coalesce, kd_server** out) {
+  if (!trees || ntrees == 0 || !out) return kd::set_error(KD_ERR_INVALID, "null argument");
+  if (slots == 0 || slots > 8) return kd::set_error(KD_ERR_INVALID, "slots must be in [1, 8]");
+  if (arena_bytes < (1u << 20)) return kd::set_error(KD_ERR_INVALID, "arena below 1 MiB");
+  auto* S = new kd_server;
+  S->trees.assign(trees, trees + ntrees);
+  for (kd_tree* t : S->trees)
+    if (!t) {
+      delete S;
+      return kd::set_error(KD_ERR_INVALID, "null tree");
+    }
+  S->slots = slots;
+  S->max_coalesce = max_coalesce ? max_coalesce : (uint64_t(1) << 18);
+  (void)hipGetDevice(&S->device);
+  S->arena.init(arena_bytes);
+  int rc = KD_OK;
+  if (hipMalloc(&S->dev, arena_bytes) != hipSuccess) rc = kd::set_error(KD_ERR_NO_MEMORY, "device arena");
+  if (!rc && hipHostMalloc(&S->pinned, arena_bytes, hipHostMallocDefault) != hipSuccess)
+    rc = kd::set_error(KD_ERR_NO_MEMORY, "pinned staging");
+  S->streams.resize(slots, nullptr);
+  S->events.resize(slots, nullptr);
+  S->slot_busy.assign(slots, false);
+  for (uint32_t j = 0; j < slots && !rc; ++j) {
+    if (hipStreamCreateWithFlags(&S->streams[j], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&S->events[j], hipEventDisableTiming) != hipSuccess)
+      rc = kd::set_error(KD_ERR_DEVICE, "stream/event creation failed");
+  }
+  if (rc) {
+    for (auto s : S->streams)
+      if (s) (void)hipStreamDestroy(s);
+    for (auto e : S->events)
+      if (e) (void)hipEventDestroy(e);
+    if (S->dev) (void)hipFree(S->dev);
+    if (S->pinned) (void)hipHostFree(S->pinned);
+    delete S;
+    return rc;
+  }
+  S->loop = std::thread(kd::serve, S);
+  *out = S;
+  return KD_OK;
+}
+
+int kd_server_submit(kd_server* S, const kd_request* r, uint64_t* ticket) {
+  if (!S || !r || !ticket) return kd::set_error(KD_ERR_INVALID, "null argument");
+  if (r->tree < 0 || uint32_t(r->tree) >= S->trees.size()) return kd::set_error(KD_ERR_INVALID, "bad tree index");
+  if (r->m == 0 || !r->queries) return kd::set_error(KD_ERR_INVALID, "empty request");
+  if ((r->family == KD_FAMILY_NEAREST_N || r->family == KD_FAMILY_BEST_N_WITHIN) &&
+      (r->k == 0 || r->k > uint32_t(kd::kMaxK)))
+    return kd::set_error(KD_ERR_INVALID, "k out of range");
+  if (r->family < KD_FAMILY_NEAREST_ONE || r->family > KD_FAMILY_APPROX)
+    return kd::set_error(KD_ERR_UNSUPPORTED, "family %d is not served", r->family);
+  if (kd::launch_bytes(S->trees[r->tree], *r, r->m) > S->arena.size)
+    return kd::set_error(KD_ERR_CAPACITY, "request needs more result memory than the arena holds");
+  auto* R = new kd::Req;
+  R->r = *r;
+  R->submitted = kd::Clock::now();
+  {
+    std::lock_guard<std::mutex> g(S->mu);
+    R->ticket = S->next_ticket++;
+    S->reqs[R->ticket] = R;
+    S->submitted.push_back(R);
+  }
+  S->cv_work.notify_one();
+  *ticket = R->ticket;
+  return KD_OK;
+}
+
+int kd_server_wait(kd_server* S, uint64_t ticket, double* latency_ms) {
+  if (!S) return kd::set_error(KD_ERR_INVALID, "server is null");
+  std::unique_lock<std::mutex> g(S->mu);
+  auto it = S->reqs.find(ticket);
+  if (it == S->reqs.end()) return kd::set_error(KD_ERR_INVALID, "unknown ticket");
+  kd::Req* R = it->second;
+  S->cv_done.wait(g, [&] { return R->done; });
+  const int status = R->status;
+  if (latency_ms) *latency_ms = R->latency_ms;
+  S->reqs.erase(it);
+  delete R;
+  return status;
+}
+
+int kd_server_stats(kd_server* S, uint64_t* out6) {
+  if (!S || !out6) return kd::set_error(KD_ERR_INVALID, "null argument");
+  out6[0] = S->st_done;
+  out6[1] = S->st_launches;
+  out6[2] = S->st_queries;
+  out6[3] = S->st_coalesced;
+  out6[4] = S->st_mem_waits;
+  std::lock_guard<std::mutex> g(S->mu);
+  out6[5] = S->peak_in_use;
+  return KD_OK;
+}
+
+int kd_server_destroy(kd_server* S) {
+  if (!S) return KD_OK;
+  {
+    std::lock_guard<std::mutex> g(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  if (S->loop.joinable()) S->loop.join();
+  for (auto s : S->streams) (void)hipStreamDestroy(s);
+  for (auto e : S->events) (void)hipEventDestroy(e);
+  (void)hipFree(S->dev);
+  (void)hipHostFree(S->pinned);
+  for (auto& kv : S->reqs) delete kv.second;
+  delete S;
+  return KD_OK;
+}
+
+}  // extern "C"

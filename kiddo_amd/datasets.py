@@ -71,3 +71,13 @@ def palette_u16(n: int = 65536, dims: int = 4, seed: int = 0) -> np.ndarray:
     """n colours with u16 channels in [0, 65534] (65535 is Kiddo's no-pivot sentinel)."""
     rng = np.random.default_rng(seed)
     return rng.integers(0, 65535, size=(n, dims), dtype=np.uint16)
+
+
+def weight_rank_items(n: int, seed: int = 0) -> np.ndarray:
+    """Item ids that encode a seeded per-item f32 weight: item = rank of the weight, so the
+    smallest item (Kiddo's best_n_within "best") is the entry with the smallest weight."""
+    w = np.random.default_rng(seed).random(n, dtype=np.float32)
+    order = np.argsort(w, kind="stable")
+    items = np.empty(n, np.uint32)
+    items[order] = np.arange(n, dtype=np.uint32)
+    return items
