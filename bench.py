@@ -57,10 +57,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
     ap.add_argument("--no-latency", action="store_true")
-    ap.add_argument("--workload", choices=["knn16", "forest", "serving"], default="knn16",
+    ap.add_argument("--workload", choices=["knn16", "forest", "serving", "mixture", "small"],
+                    default="knn16",
                     help="knn16: the headline 16M-point k-NN; forest: 4096-tree batched form; "
                          "serving: continuous batching over resident trees")
     ap.add_argument("--requests", type=int, default=3000)
+    ap.add_argument("--result-budget", type=int, default=8 << 30,
+                    help="mixture: bytes of within results held at once (queries run in chunks)")
     ap.add_argument("--rate", type=float, default=4000.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--trees", type=int, default=4096)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -84,6 +87,10 @@ def main():
         return bench_forest(args, torch, ka)
     if args.workload == "serving":
         return bench_serving(args, torch, ka)
+    if args.workload == "mixture":
+        return bench_mixture(args, torch, ka)
+    if args.workload == "small":
+        return bench_small(args, torch, ka)
 
     if args.copy_probe:
         # A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it scales the
@@ -435,6 +442,120 @@ def bench_serving(args, torch, ka):
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "per_family": by_fam, "server": st,
         "offered_qps": round(total_q / arrivals[-1], 1),
+    }), flush=True)
+
+
+def _time_knn(tree, q, k, flags, steps, warmup, torch, out=None):
+    for _ in range(warmup):
+        out = tree.nearest_n_raw(q, k, math.inf, flags, out=out)
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = tree.nearest_n_raw(q, k, math.inf, flags, out=out)
+        kms.append(tree.last_kernel_ms())
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, float(np.mean(kms)), out
+
+
+def bench_mixture(args, torch, ka):
+    """ImmutableKdTree<f32, 3>: 10,000,000 points from a 64-component isotropic Gaussian
+    mixture in the unit cube (centres uniform, sigma 0.01, clipped), bucket 32; 10,000,000
+    queries from the same mixture; nearest_n(16), and within(r = 0.005, passed as r^2 =
+    2.5e-5) unsorted, whose ~1.8e10 results are produced in query chunks that fit
+    `--result-budget` bytes (count pass, then one fill pass per chunk)."""
+    from kiddo_amd.datasets import gaussian_mixture
+    n = m = 10_000_000
+    centres = np.random.default_rng(args.seed ^ 0x5EED).random((64, 3))
+    pts = gaussian_mixture(n, 64, 0.01, 3, seed=args.seed, centres=centres)
+    qh = gaussian_mixture(m, 64, 0.01, 3, seed=args.seed + 1, centres=centres)
+    tree = ka.ImmutableKdTree.new_from_slice(pts, args.bucket)
+    tree.set_timing(True)
+    q = torch.from_numpy(qh).cuda()
+    flags = ka.KD_MORTON
+    chosen = tree.autotune_nearest_n(q, 16, math.inf, flags)
+    dt, kms, _ = _time_knn(tree, q, 16, flags, args.steps, args.warmup, torch)
+    st = tree.query_stats()
+    log(f"mixture knn16: {dt * 1e3:.1f} ms/step ({m / dt / 1e6:.1f} Mq/s), kernel {kms:.1f} ms, {st}")
+    # within, chunked by result memory
+    r2 = 0.005 ** 2
+    wflags = ka.KD_UNSORTED | ka.KD_MORTON
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    offs = torch.empty(m + 1, dtype=torch.int64, device="cuda")
+    ka._check(ka.lib().kd_within_count(tree.handle, q.data_ptr(), m, r2, wflags, offs.data_ptr(),
+                                       ka._stream_ptr(None)), "kd_within_count")
+    offs_h = offs.cpu().numpy()
+    t_count = time.perf_counter() - t0
+    total = int(offs_h[-1])
+    cap = max(1, args.result_budget // 8)  # u32 item + f32 distance per result
+    items = torch.empty(min(cap, max(total, 1)), dtype=torch.uint32, device="cuda")
+    dists = torch.empty(min(cap, max(total, 1)), dtype=torch.float32, device="cuda")
+    a, chunks = 0, 0
+    while a < m:
+        b = int(np.searchsorted(offs_h, offs_h[a] + cap, side="right")) - 1
+        b = max(min(b, m), a + 1)
+        sub = (offs[a:b + 1] - offs[a]).contiguous()
+        ka._check(ka.lib().kd_within_fill(tree.handle, q[a:b].data_ptr(), b - a, r2, wflags,
+                                          sub.data_ptr(), items.data_ptr(), dists.data_ptr(),
+                                          ka._stream_ptr(None)), "kd_within_fill")
+        chunks += 1
+        a = b
+    torch.cuda.synchronize()
+    t_within = time.perf_counter() - t0
+    log(f"mixture within: {total} results in {chunks} chunks, count {t_count * 1e3:.0f} ms, "
+        f"total {t_within * 1e3:.0f} ms ({total / t_within / 1e9:.2f} G results/s)")
+    algo = st["points"] * 16 + st["stems"] * 4 + m * 12 + m * 16 * 8
+    print(json.dumps({
+        "metric": "queries/s, 10M-pt f32 Gaussian-mixture kNN k=16; within(r=0.005) results/s",
+        "value": round(m / dt, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded mixture)",
+        "config": {"workload": "mixture-10M-f32-knn16+within0.005", "points": n, "queries": m,
+                   "k": 16, "launch": chosen},
+        "within_r0.005_unsorted": {"results": total, "chunks": chunks,
+                                   "result_budget_bytes": args.result_budget,
+                                   "count_ms": round(t_count * 1e3, 1),
+                                   "total_ms": round(t_within * 1e3, 1),
+                                   "results_per_s": round(total / t_within, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(algo / (kms / 1e3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(algo / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_nearest_n<float,3,16>", "kernel_ms": round(kms, 3)},
+    }), flush=True)
+
+
+def bench_small(args, torch, ka):
+    """ImmutableKdTree<f64, 3>: 100,000 uniform points, bucket 32, 100,000 uniform queries;
+    nearest_one and nearest_n(8) (the reference's CPU-runnable case)."""
+    n = m = 100_000
+    pts = np.random.default_rng(args.seed).random((n, 3))
+    qh = np.random.default_rng(args.seed + 1).random((m, 3))
+    tree = ka.ImmutableKdTree.new_from_slice(pts, args.bucket)
+    tree.set_timing(True)
+    q = torch.from_numpy(qh).cuda()
+    res = {}
+    for name, k in (("nearest_n8", 8),):
+        dt, kms, _ = _time_knn(tree, q, k, ka.KD_MORTON, max(args.steps, 20), args.warmup, torch)
+        res[name] = {"ms": round(dt * 1e3, 4), "kernel_ms": round(kms, 4), "qps": round(m / dt, 1)}
+    for _ in range(args.warmup):
+        tree.nearest_one_raw(q, ka.KD_MORTON)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(args.steps, 20)):
+        tree.nearest_one_raw(q, ka.KD_MORTON)
+    torch.cuda.synchronize()
+    dt1 = (time.perf_counter() - t0) / max(args.steps, 20)
+    res["nearest_one"] = {"ms": round(dt1 * 1e3, 4), "qps": round(m / dt1, 1)}
+    log(f"small: {res}")
+    print(json.dumps({
+        "metric": "queries/s, 100k-pt f64 3D nearest_one and nearest_n(8)",
+        "value": res["nearest_n8"]["qps"], "unit": "queries/s", "n_gpus": 1,
+        "steps": max(args.steps, 20), "warmup": args.warmup, "ms_per_step": res["nearest_n8"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded uniform)", "config": {"workload": "small-100k-f64", "points": n,
+                                                         "queries": m},
+        "families": res,
     }), flush=True)
 
 
