@@ -63,19 +63,18 @@ template <class O> __device__ __forceinline__ O abs_diff(O a, O b) {
   return a >= b ? sub_rn(a, b) : sub_rn(b, a);
 }
 
-// Register-resident small arrays indexed by a run-time dimension.
-template <class T, int K> __device__ __forceinline__ T pick(const T (&a)[K], int d) {
-  T r = a[0];
-#pragma unroll
-  for (int j = 1; j < K; ++j)
-    if (d == j) r = a[j];
-  return r;
-}
-template <class T, int K> __device__ __forceinline__ void put(T (&a)[K], int d, T v) {
-#pragma unroll
-  for (int j = 0; j < K; ++j)
-    if (d == j) a[j] = v;
-}
+// A K-vector (K <= 4) indexed by a run-time dimension. It is held as one vector value, so
+// an element access is an extract/insert on registers that the backend expands into
+// compare-and-select; named scalars or an array let the optimiser fold the selects back
+// into an address computation on a stack slot, which spills the vector to scratch memory.
+template <class T, int K> struct Small {
+  static_assert(K >= 1 && K <= 4, "K must be 1..4");
+  typedef T vec4 __attribute__((ext_vector_type(4)));
+  vec4 v;
+  __device__ __forceinline__ T operator[](int d) const { return v[d]; }
+  __device__ __forceinline__ void set(int d, T x) { v[d] = x; }
+  __device__ __forceinline__ void fill(T x) { v = vec4(x); }
+};
 
 constexpr int kMaxLevels = 32;
 
@@ -108,9 +107,8 @@ template <class A, int K> __device__ __forceinline__ void pin_stems(DevTree<A, K
 }
 
 // Squared distance of entry p in Kiddo's operation order.
-template <class A, int K>
-__device__ __forceinline__ typename Ax<A>::O point_dist(const DevTree<A, K>& t, uint32_t p,
-                                                        const typename Ax<A>::O (&qw)[K]) {
+template <class A, int K, class QW>
+__device__ __forceinline__ typename Ax<A>::O point_dist(const DevTree<A, K>& t, uint32_t p, const QW& qw) {
   using O = typename Ax<A>::O;
   O acc = O(0);
 #pragma unroll
@@ -131,15 +129,20 @@ struct Counters {
 // ------------------------------------------------------------------ traversal --------
 // V supplies bound(), prune(rd), unbounded() and leaf(tree, qw, leaf_index).
 template <class A, int K, class V>
-__device__ __forceinline__ void traverse(const DevTree<A, K>& t, const A (&q)[K],
-                                         const typename Ax<A>::O (&qw)[K], V& vis, Counters& c) {
+__device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, K>& q,
+                                         const Small<typename Ax<A>::O, K>& qw, V& vis, Counters& c) {
+  // Register-only form of Kiddo's DFS: no per-level stack. A queued far child at level l is
+  // a `pending` bit; its rect distance rd' = (rd - off[dim]^2) + new_off^2 is recomputed
+  // when the bit is popped, from the same rd and off[] it was computed from at push time
+  // (nothing below l has been entered then) and the same pivot, so it is bit-identical.
+  // Entering far children is recorded in `taken`; rd and off[] are a function of the taken
+  // levels along the current path, so popping a taken level rebuilds them by replaying the
+  // remaining taken levels top-down, with the exact operations used when they were entered.
   using O = typename Ax<A>::O;
   if (t.n == 0) return;  // orchestrator/mod.rs:399-401
   const int D = t.nlev;
-  O off[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) off[j] = O(0);
-  O slot[kMaxLevels];   // far rd while pending, saved off[dim] while inside the far child
+  Small<O, K> off;
+  off.fill(O(0));
   uint32_t pending = 0, taken = 0;
   O rd = O(0);
   uint32_t idx = 1;
@@ -151,14 +154,14 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const A (&q)[K]
     while (level < D) {
       const A pivot = stem_at(t, idx);
       if (pivot < Ax<A>::maxv()) {
-        const bool right = pick(q, dim) >= pivot;
-        const O qd = pick(qw, dim);
-        const O no = abs_diff(qd, O(pivot));
-        const O oo = pick(off, dim);
-        const O rdf = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
-        if (!check || !(rdf > vis.bound())) {
-          slot[level] = rdf;
+        const bool right = q[dim] >= pivot;
+        if (!check) {
           pending |= 1u << level;
+        } else {
+          const O no = abs_diff(qw[dim], O(pivot));
+          const O oo = off[dim];
+          const O rdf = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+          if (!(rdf > vis.bound())) pending |= 1u << level;
         }
         idx = (idx << 1) | (right ? 1u : 0u);
       } else {
@@ -178,21 +181,35 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const A (&q)[K]
     while (pending | taken) {
       const int l = 31 - __clz(pending | taken);
       const uint32_t bit = 1u << l;
-      const int ld = l % K;
       if (taken & bit) {
-        put(off, ld, slot[l]);
         taken ^= bit;
+        // rebuild rd and off[] from the remaining taken levels, top-down
+        rd = O(0);
+        off.fill(O(0));
+        uint32_t rest = taken;
+        while (rest) {
+          const int tl = __ffs(rest) - 1;
+          rest &= rest - 1u;
+          const int td = tl % K;
+          const A pv = stem_at(t, idx >> (D - tl));
+          c.stems += 1;
+          const O no = abs_diff(qw[td], O(pv));
+          const O oo = off[td];
+          rd = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+          off.set(td, no);
+        }
         continue;
       }
       pending ^= bit;
-      const O frd = slot[l];
-      if (vis.prune(frd)) continue;
+      const int ld = l % K;
       const A pivot = stem_at(t, idx >> (D - l));
       c.stems += 1;
-      const O no = abs_diff(pick(qw, ld), O(pivot));
-      slot[l] = pick(off, ld);
+      const O no = abs_diff(qw[ld], O(pivot));
+      const O oo = off[ld];
+      const O frd = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+      if (vis.prune(frd)) continue;
       taken |= bit;
-      put(off, ld, no);
+      off.set(ld, no);
       rd = frd;
       idx = (idx >> (D - l - 1)) ^ 1u;  // far child of the level-l ancestor
       level = l + 1;
@@ -216,8 +233,8 @@ template <class O> struct NearestOneVis {
   __device__ O bound() const { return best; }
   __device__ bool prune(O rd) const { return rd > best; }
   __device__ bool unbounded() const { return true; }
-  template <class A, int K>
-  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     for (uint32_t p = s; p < e; ++p) {
       const O d = point_dist(t, p, qw);
@@ -307,8 +324,8 @@ template <class O, int MAXK> struct KnnVis {
     cnt = cnt < k ? cnt + 1 : k;
     if (cnt >= k) top.kth(k, kd, kit);
   }
-  template <class A, int K>
-  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     for (uint32_t p = s; p < e; ++p) {
       const O d = point_dist(t, p, qw);
@@ -334,8 +351,8 @@ template <class O> struct WithinVis {
   __device__ O bound() const { return radius; }
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
-  template <class A, int K>
-  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     for (uint32_t p = s; p < e; ++p) {
       const O d = point_dist(t, p, qw);
@@ -374,8 +391,8 @@ template <class O, int MAXK> struct BestNVis {
   __device__ O bound() const { return radius; }
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
-  template <class A, int K>
-  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     for (uint32_t p = s; p < e; ++p) {
       const O dd = point_dist(t, p, qw);
@@ -439,8 +456,8 @@ template <int MAXK> struct ScreenKnnVis {
   // against the threshold at the start of the run, then only survivors are measured in
   // FP64 and offered (in entry order) to the collector. The wave then pays the FP64 and
   // insertion work once per survivor of its busiest lane, not once per scanned entry.
-  template <int K>
-  __device__ void leaf(const DevTree<double, K>& t, const double (&qw)[K], uint32_t leaf) {
+  template <int K, class QW>
+  __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf) {
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     for (uint32_t c0 = s; c0 < e; c0 += 32) {
       const uint32_t n = (e - c0) < 32u ? (e - c0) : 32u;
@@ -491,13 +508,16 @@ __device__ __forceinline__ void flush_counters(const Counters& c, unsigned long 
 }
 
 template <class A, int K>
-__device__ __forceinline__ void load_query(const A* __restrict__ queries, uint64_t qi, A (&q)[K],
-                                           typename Ax<A>::O (&qw)[K]) {
+__device__ __forceinline__ void load_query(const A* __restrict__ queries, uint64_t qi, Small<A, K>& q,
+                                           Small<typename Ax<A>::O, K>& qw) {
   using O = typename Ax<A>::O;
+  q.fill(A(0));
+  qw.fill(O(0));
 #pragma unroll
   for (int d = 0; d < K; ++d) {
-    q[d] = queries[qi * K + d];
-    qw[d] = O(q[d]);
+    const A v = queries[qi * K + d];
+    q.set(d, v);
+    qw.set(d, O(v));
   }
 }
 
@@ -511,8 +531,8 @@ constexpr int kBlock = 256;
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;     \
   const bool active = i < m;                                          \
   const uint64_t qi = active ? (order ? order[i] : i) : 0;            \
-  A q[K];                                                             \
-  O qw[K];                                                            \
+  Small<A, K> q;                                                      \
+  Small<O, K> qw;                                                     \
   if (active) load_query<A, K>(queries, qi, q, qw);                   \
   Counters cnt;
 
@@ -549,15 +569,14 @@ __device__ __forceinline__ O gs_read(const O (&reg)[GroupStack<G>::R], int l) {
 // The traversal of `traverse` with the stack spread over the group's lanes. V::leaf is
 // called by every lane of the group.
 template <class A, int K, int G, class V>
-__device__ __forceinline__ void group_traverse(const DevTree<A, K>& t, const A (&q)[K],
-                                               const typename Ax<A>::O (&qw)[K], V& vis,
+__device__ __forceinline__ void group_traverse(const DevTree<A, K>& t, const Small<A, K>& q,
+                                               const Small<typename Ax<A>::O, K>& qw, V& vis,
                                                Counters& c, int gl) {
   using O = typename Ax<A>::O;
   if (t.n == 0) return;
   const int D = t.nlev;
-  O off[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) off[j] = O(0);
+  Small<O, K> off;
+  off.fill(O(0));
   O reg[GroupStack<G>::R];
 #pragma unroll
   for (int j = 0; j < GroupStack<G>::R; ++j) reg[j] = O(0);
@@ -571,9 +590,9 @@ __device__ __forceinline__ void group_traverse(const DevTree<A, K>& t, const A (
     while (level < D) {
       const A pivot = stem_at(t, idx);
       if (pivot < Ax<A>::maxv()) {
-        const bool right = pick(q, dim) >= pivot;
-        const O no = abs_diff(pick(qw, dim), O(pivot));
-        const O oo = pick(off, dim);
+        const bool right = q[dim] >= pivot;
+        const O no = abs_diff(qw[dim], O(pivot));
+        const O oo = off[dim];
         const O rdf = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
         if (!check || !(rdf > vis.bound())) {
           gs_write<O, G>(reg, gl, level, rdf);
@@ -598,7 +617,7 @@ __device__ __forceinline__ void group_traverse(const DevTree<A, K>& t, const A (
       const uint32_t bit = 1u << l;
       const int ld = l % K;
       if (taken & bit) {
-        put(off, ld, gs_read<O, G>(reg, l));
+        off.set(ld, gs_read<O, G>(reg, l));
         taken ^= bit;
         continue;
       }
@@ -607,10 +626,10 @@ __device__ __forceinline__ void group_traverse(const DevTree<A, K>& t, const A (
       if (vis.prune(frd)) continue;
       const A pivot = stem_at(t, idx >> (D - l));
       c.stems += 1;
-      const O no = abs_diff(pick(qw, ld), O(pivot));
-      gs_write<O, G>(reg, gl, l, pick(off, ld));
+      const O no = abs_diff(qw[ld], O(pivot));
+      gs_write<O, G>(reg, gl, l, off[ld]);
       taken |= bit;
-      put(off, ld, no);
+      off.set(ld, no);
       rd = frd;
       idx = (idx >> (D - l - 1)) ^ 1u;
       level = l + 1;
@@ -681,7 +700,8 @@ template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {
       if (SCREEN) thr = screen_threshold(double(bound()), eps_k);
     }
   }
-  __device__ void leaf(const DevTree<A, K>& t, const O (&qw)[K], uint32_t leaf) {
+  template <class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     const int base = (threadIdx.x & 63) & ~(G - 1);
     for (uint32_t c0 = s; c0 < e; c0 += G) {

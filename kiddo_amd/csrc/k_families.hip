@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_approx_nearest_one(DevTree<A, K> t, 
       int dim = 0;
       for (int level = 0; level < t.nlev; ++level) {
         const A pivot = stem_at(t, idx);
-        idx = (idx << 1) | (pick(q, dim) >= pivot ? 1u : 0u);
+        idx = (idx << 1) | (q[dim] >= pivot ? 1u : 0u);
         dim = (dim + 1 == K) ? 0 : dim + 1;
       }
       const uint32_t leaf = idx - (1u << t.nlev);

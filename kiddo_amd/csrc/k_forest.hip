@@ -104,8 +104,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     for (uint32_t j = threadIdx.x; j < wi.count; j += kBlock) {
       const uint64_t pos = uint64_t(wi.begin) + j;
       const uint64_t qi = order ? order[pos] : pos;
-      A q[K];
-      O qw[K];
+      Small<A, K> q;
+      Small<O, K> qw;
       load_query<A, K>(queries, qi, q, qw);
       KnnVis<O, MAXK> v;
       v.init(k, radius, excl != 0);
@@ -145,8 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_forest_knn_group(
     for (uint32_t j = slot; j < wi.count; j += per_block) {  // uniform within a group
       const uint64_t pos = uint64_t(wi.begin) + j;
       const uint64_t qi = order ? order[pos] : pos;
-      A q[K];
-      O qw[K];
+      Small<A, K> q;
+      Small<O, K> qw;
       load_query<A, K>(queries, qi, q, qw);
       GroupKnnVis<A, K, G, false> v;
       v.init(k, radius, excl != 0, gl);
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(kBlock) void k_forest_within(
     for (uint32_t j = threadIdx.x; j < wi.count; j += kBlock) {
       const uint64_t pos = uint64_t(wi.begin) + j;
       const uint64_t qi = order ? order[pos] : pos;
-      A q[K];
-      O qw[K];
+      Small<A, K> q;
+      Small<O, K> qw;
       load_query<A, K>(queries, qi, q, qw);
       if (offsets) {
         const uint64_t o = offsets[qi], cap = offsets[qi + 1] - o;

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_nearest_n(DevTree<A, K> t, const A* 
 // Per-query FP32 screen set-up: the query rounded to FP32, and sqrt(K) times the bound on
 // the error of an FP32 coordinate difference (see screen_threshold).
 template <int K>
-__device__ __forceinline__ void screen_setup(const double (&q)[K], const double* __restrict__ max_abs,
+__device__ __forceinline__ void screen_setup(const Small<double, K>& q, const double* __restrict__ max_abs,
                                              float (&qf)[3], double& eps_k) {
   double amax = 0.0;
 #pragma unroll
@@ -100,8 +100,8 @@ __global__ __launch_bounds__(kBlock) void k_knn_group(DevTree<A, K> t, const A* 
   const int gl = int(threadIdx.x) & (G - 1);
   const bool active = gid < m;  // uniform within a group
   const uint64_t qi = active ? (order ? order[gid] : gid) : 0;
-  A q[K];
-  O qw[K];
+  Small<A, K> q;
+  Small<O, K> qw;
   Counters cnt;
   if (active) {
     load_query<A, K>(queries, qi, q, qw);
