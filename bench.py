@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--no-within", action="store_true", help="knn16: skip the within(r=0.002) pass")
     ap.add_argument("--workload", choices=["knn16", "forest", "serving", "mixture", "small"],
                     default="knn16",
                     help="knn16: the headline 16M-point k-NN; forest: 4096-tree batched form; "
@@ -214,6 +215,29 @@ def main():
                     kernel_ms=round(results[head]["kernel_ms"], 3),
                     algo_bytes_per_launch=int(algo_bytes))
 
+    # compulsory footprint: tree bytes + queries + results, once each, over the kernel time
+    footprint = (tree.info.device_bytes + m * 3 * 8 + m * k * (4 + 8) + m * 4)
+    compulsory = dict(bytes=int(footprint), gbs=round(footprint / kern_s / 1e9, 1))
+
+    # within(r = 0.002), sorted: count pass + scan + fill pass, same tree and queries
+    within = None
+    if not args.no_within:
+        r2 = 0.002 * 0.002
+        tree.within_raw(q[:4096], r2, ka.KD_MORTON)  # warm the within kernels
+        torch.cuda.synchronize()
+        wt = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            w = tree.within_raw(q, r2, ka.KD_MORTON)
+            torch.cuda.synchronize()
+            wt.append(time.perf_counter() - t0)
+        nres = int(w.offsets[-1].item())
+        wdt = float(np.median(wt))
+        within = dict(ms=round(wdt * 1e3, 2), results=nres, order="Morton", results_per_query=round(nres / m, 3),
+                      queries_per_s=round(m / wdt, 1))
+        log(f"within r=0.002 sorted: {wdt * 1e3:.1f} ms, {nres} results")
+        del w
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -223,12 +247,16 @@ def main():
                                 lay.stems(), lay.extents(), lay.arena())
         ns = min(args.cpu_sample, m)
         qs = np.ascontiguousarray(qh[:ns])
-        t0 = time.perf_counter()
-        ci, cd, cc = ot.nearest_n(qs, k, mode=O.FAITHFUL, threads=threads)
-        dt = time.perf_counter() - t0
+        runs = []
+        for _ in range(3):  # median of three runs
+            t0 = time.perf_counter()
+            ci, cd, cc = ot.nearest_n(qs, k, mode=O.FAITHFUL, threads=threads)
+            runs.append(time.perf_counter() - t0)
+        dt = float(np.median(runs))
         cpu = dict(value=round(ns / dt, 1), unit="queries/s", cores=threads, kind="port",
                    sample=f"first {ns} of the {m} queries, nearest_n(k=16), Kiddo's scalar "
-                          f"query restated in C++ (oracle/kiddo_oracle.cpp), {threads} threads",
+                          f"query restated in C++ (oracle/kiddo_oracle.cpp), {threads} threads, "
+                          f"median of 3 runs",
                    seconds=round(dt, 2))
         # the CPU answers agree with the GPU on the sample's distances
         gd = outs[head].distances[:ns].cpu().numpy()
@@ -262,6 +290,8 @@ def main():
         "fp32_screen_error": screen_err,
         "leaves_per_query": round(st["leaves"] / m, 2),
         "entries_per_query": round(st["points"] / m, 2),
+        "compulsory_footprint": compulsory,
+        "within_r0.002_sorted": within,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
