@@ -165,6 +165,35 @@ int kd_best_n_within(kd_tree* tree, const void* queries, uint64_t m, double radi
                      uint32_t n, uint32_t flags, uint32_t* items, void* dists,
                      uint32_t* counts, void* stream);
 
+/* ------------------------------------------------ periodic boundary conditions ---- */
+/* tree.query(q).periodic_boundary_condition(&box)...  (builder.rs:1280-1291, 1989-2003;
+ * query/periodic.rs). box: K strictly positive lengths in the tree's coordinate type
+ * (host memory; builder.rs:29-35 rejects a length <= 0 or NaN: KD_ERR_INVALID). Float
+ * trees only (PeriodicAxis is implemented for f32/f64: KD_ERR_UNSUPPORTED otherwise).
+ * Distances of nearest_n / within are minimum-image distances (per axis the smaller of
+ * |q - c| and L - |q - c|); best_n_within is Cartesian-only in the reference. */
+
+/* .nearest_one() (periodic.rs:196-244): Cartesian nearest_one of the home query, then of
+ * the shifted query images (every axis -L, 0 or +L) whose lower bound does not exceed the
+ * best distance, in ascending lower bound; dists in A, each being the Cartesian distance
+ * to the image that produced the result. Ties: lower item. */
+int kd_periodic_nearest_one(kd_tree* tree, const void* queries, uint64_t m, const void* box,
+                            uint32_t flags, uint32_t* items, void* dists, void* stream);
+
+/* .nearest_n(k)[.within(radius)][.exclusive_boundaries()][.unsorted()]
+ * (periodic.rs:620-791). Outputs as kd_nearest_n. */
+int kd_periodic_nearest_n(kd_tree* tree, const void* queries, uint64_t m, const void* box,
+                          uint32_t k, double radius, uint32_t flags, uint32_t* items, void* dists,
+                          uint32_t* counts, void* stream);
+
+/* .within(radius)[.unsorted()][.exclusive_boundaries()], phases 1 and 2, as
+ * kd_within_count / kd_within_fill (periodic.rs:620-791 with the Vec collector). */
+int kd_periodic_within_count(kd_tree* tree, const void* queries, uint64_t m, const void* box,
+                             double radius, uint32_t flags, uint64_t* offsets, void* stream);
+int kd_periodic_within_fill(kd_tree* tree, const void* queries, uint64_t m, const void* box,
+                            double radius, uint32_t flags, const uint64_t* offsets,
+                            uint32_t* items, void* dists, void* stream);
+
 /* Traversal counters of the last query call on this tree (synchronous; calls that
  * overlap on one tree share them): out4 = {leaves visited, entries scanned, stem nodes
  * read, entries re-measured in FP64 by KD_SCREEN_F32}. The first three equal what Kiddo's

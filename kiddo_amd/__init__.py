@@ -84,6 +84,10 @@ SIGNATURES = {
     "kd_within_count": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp], C.c_int),
     "kd_within_fill": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_best_n_within": ([_vp, _vp, _u64, _f64, _u32, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_periodic_nearest_one": ([_vp, _vp, _u64, _vp, _u32, _vp, _vp, _vp], C.c_int),
+    "kd_periodic_nearest_n": ([_vp, _vp, _u64, _vp, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_periodic_within_count": ([_vp, _vp, _u64, _vp, _f64, _u32, _vp, _vp], C.c_int),
+    "kd_periodic_within_fill": ([_vp, _vp, _u64, _vp, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_query_stats": ([_vp, _vp], C.c_int),
     "kd_forest_create": ([_vp, _u32, _u32, C.POINTER(_vp)], C.c_int),
     "kd_forest_destroy": ([_vp], C.c_int),
@@ -407,6 +411,54 @@ class ImmutableKdTree:
         return out
 
 
+    # -- periodic boundary conditions (query/periodic.rs) --------------------------------
+    def _box(self, box):
+        b = np.ascontiguousarray(box, dtype=self.a_dtype)
+        if b.shape != (self.K,):
+            raise ValueError(f"box must hold {self.K} lengths")
+        return b
+
+    def periodic_nearest_one_raw(self, q, box, flags=0, stream=None):
+        torch = _torch()
+        q, b = self._queries(q), self._box(box)
+        m = q.shape[0]
+        items = torch.empty(m, dtype=torch.uint32, device="cuda")
+        d = self._out(m)
+        _check(lib().kd_periodic_nearest_one(self._h, q.data_ptr(), m, _np_ptr(b), flags,
+                                             items.data_ptr(), d.data_ptr(), _stream_ptr(stream)),
+               "kd_periodic_nearest_one")
+        return NearestOne(items, d)
+
+    def periodic_nearest_n_raw(self, q, box, k, radius=math.inf, flags=0, stream=None):
+        torch = _torch()
+        q, b = self._queries(q), self._box(box)
+        m = q.shape[0]
+        out = NearestN(torch.empty((m, k), dtype=torch.uint32, device="cuda"),
+                       self._out(m, k), torch.empty(m, dtype=torch.uint32, device="cuda"))
+        _check(lib().kd_periodic_nearest_n(self._h, q.data_ptr(), m, _np_ptr(b), k, float(radius),
+                                           flags, out.items.data_ptr(), out.distances.data_ptr(),
+                                           out.counts.data_ptr(), _stream_ptr(stream)),
+               "kd_periodic_nearest_n")
+        return out
+
+    def periodic_within_raw(self, q, box, radius, flags=0, stream=None):
+        torch = _torch()
+        q, b = self._queries(q), self._box(box)
+        m = q.shape[0]
+        offsets = torch.empty(m + 1, dtype=torch.int64, device="cuda")
+        sp = _stream_ptr(stream)
+        _check(lib().kd_periodic_within_count(self._h, q.data_ptr(), m, _np_ptr(b), float(radius),
+                                              flags, offsets.data_ptr(), sp),
+               "kd_periodic_within_count")
+        total = int(offsets[m].item()) if m else 0
+        items = torch.empty(max(total, 1), dtype=torch.uint32, device="cuda")
+        d = self._out(max(total, 1))
+        _check(lib().kd_periodic_within_fill(self._h, q.data_ptr(), m, _np_ptr(b), float(radius),
+                                             flags, offsets.data_ptr(), items.data_ptr(),
+                                             d.data_ptr(), sp), "kd_periodic_within_fill")
+        return Within(offsets, items[:total], d[:total])
+
+
 class Forest:
     """Many independent trees (same dtype and K) in pooled HBM buffers, queried as one batch
     by a persistent kernel fed from a device work queue (kd_forest_* in kiddo_amd.h)."""
@@ -580,11 +632,19 @@ class QueryBuilder:
         self.radius = math.inf
         self.flags = 0
         self.is_approx = False
+        self.box = None
 
     def _family(self, fam):
         if self.family is not None:
             raise ValueError("query family already chosen")
         self.family = fam
+        return self
+
+    # space (builder.rs:1280-1291, 1989-2003): chosen before the family
+    def periodic_boundary_condition(self, box_size):
+        if self.family is not None:
+            raise ValueError("periodic_boundary_condition() must come before the query family")
+        self.box = box_size
         return self
 
     # families (builder.rs:2008-2057)
@@ -605,6 +665,8 @@ class QueryBuilder:
         return self._family("within")
 
     def best_n_within(self, radius: float, n: int):
+        if self.box is not None:
+            raise ValueError("best_n_within is Cartesian-only (builder.rs:1287-1290)")
         if n < 1:
             raise ValueError("max_qty must be non-zero")
         self.radius = radius
@@ -613,6 +675,8 @@ class QueryBuilder:
 
     # options (builder.rs:2059-2081)
     def approx(self):
+        if self.box is not None:
+            raise ValueError("approx() is not available for periodic queries")
         if self.family != "nearest_one":
             raise ValueError("approx() is only available after nearest_one()")
         self.is_approx = True
@@ -641,6 +705,16 @@ class QueryBuilder:
 
     def execute(self):
         t = self.tree
+        if self.box is not None:
+            if self.family == "nearest_one":
+                return t.periodic_nearest_one_raw(self.queries, self.box, self.flags, self.stream)
+            if self.family == "nearest_n":
+                return t.periodic_nearest_n_raw(self.queries, self.box, self.k, self.radius,
+                                                self.flags, self.stream)
+            if self.family == "within":
+                return t.periodic_within_raw(self.queries, self.box, self.radius, self.flags,
+                                             self.stream)
+            raise ValueError("choose a query family first")
         if self.family == "nearest_one":
             return t.nearest_one_raw(self.queries, self.flags, self.stream, approx=self.is_approx)
         if self.family == "nearest_n":

@@ -514,6 +514,92 @@ int kd_best_n_within(kd_tree* t, const void* queries, uint64_t m, double radius,
   return kd::launch_best_n_within(a, int(n), radius, items, dists, counts);
 }
 
+namespace kd {
+namespace {
+// builder.rs:29-35: every periodic box length must be > 0 (NaN fails); float trees only
+int check_periodic(const kd_tree* t, const void* box) {
+  if (t->dtype != KD_F32 && t->dtype != KD_F64)
+    return set_error(KD_ERR_UNSUPPORTED, "periodic queries need a float tree (PeriodicAxis is f32/f64)");
+  if (!box) return set_error(KD_ERR_INVALID, "box is null");
+  for (int d = 0; d < t->dims; ++d) {
+    const double L = t->dtype == KD_F32 ? double(static_cast<const float*>(box)[d])
+                                        : static_cast<const double*>(box)[d];
+    if (!(L > 0.0)) return set_error(KD_ERR_INVALID, "periodic box sizes must be strictly positive");
+  }
+  return KD_OK;
+}
+}  // namespace
+}  // namespace kd
+
+int kd_periodic_nearest_one(kd_tree* t, const void* queries, uint64_t m, const void* box, uint32_t flags,
+                            uint32_t* items, void* dists, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (int rc = kd::check_periodic(t, box)) return rc;
+  if (m == 0) return KD_OK;
+  if (!queries || !items || !dists) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_periodic_nearest_one(a, box, items, dists);
+}
+
+int kd_periodic_nearest_n(kd_tree* t, const void* queries, uint64_t m, const void* box, uint32_t k,
+                          double radius, uint32_t flags, uint32_t* items, void* dists, uint32_t* counts,
+                          void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (int rc = kd::check_periodic(t, box)) return rc;
+  if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1 (NonZeroUsize)");
+  if (k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "k above %d is not provided", kd::kMaxK);
+  if (int rc = kd::check_radius(radius)) return rc;
+  if (m == 0) return KD_OK;
+  if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_periodic_nearest_n(a, box, int(k), radius, items, dists, counts);
+}
+
+int kd_periodic_within_count(kd_tree* t, const void* queries, uint64_t m, const void* box, double radius,
+                             uint32_t flags, uint64_t* offsets, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (int rc = kd::check_periodic(t, box)) return rc;
+  if (!offsets) return kd::set_error(KD_ERR_INVALID, "offsets is null");
+  if (int rc = kd::check_radius(radius)) return rc;
+  if (m == 0) {
+    hipLaunchKernelGGL(kd::dev::k_set_u64, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), offsets,
+                       uint64_t(0));
+    KD_HIP(hipGetLastError());
+    return KD_OK;
+  }
+  if (!queries) return kd::set_error(KD_ERR_INVALID, "queries is null");
+  if (m >= 0x7FFFFFFFull) return kd::set_error(KD_ERR_UNSUPPORTED, "within batches hold < 2^31 queries");
+  KD_ORDERED(a, queries, m, flags, stream);
+  void* mem = nullptr;
+  size_t temp = 0;
+  uint64_t* cnt = nullptr;
+  KD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, cnt, offsets, int(m + 1), s_));
+  const size_t arr = ((m + 1) * 8 + 255) / 256 * 256;
+  KD_HIP(hipMallocAsync(&mem, arr + temp, s_));
+  cnt = static_cast<uint64_t*>(mem);
+  int rc = kd::launch_periodic_within_count(a, box, radius, cnt);
+  if (rc == KD_OK) {
+    hipLaunchKernelGGL(kd::dev::k_set_u64, dim3(1), dim3(1), 0, s_, cnt + m, uint64_t(0));
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(static_cast<uint8_t*>(mem) + arr, temp, cnt, offsets,
+                                                    int(m + 1), s_);
+    if (e != hipSuccess) rc = kd::set_error(KD_ERR_DEVICE, "scan failed: %s", hipGetErrorString(e));
+  }
+  (void)hipFreeAsync(mem, s_);
+  return rc;
+}
+
+int kd_periodic_within_fill(kd_tree* t, const void* queries, uint64_t m, const void* box, double radius,
+                            uint32_t flags, const uint64_t* offsets, uint32_t* items, void* dists,
+                            void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (int rc = kd::check_periodic(t, box)) return rc;
+  if (m == 0) return KD_OK;
+  if (!queries || !offsets) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  if (int rc = kd::check_radius(radius)) return rc;
+  KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_periodic_within_fill(a, box, radius, offsets, items, dists);
+}
+
 int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double radius,
                           uint32_t flags, int32_t* chosen4, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;

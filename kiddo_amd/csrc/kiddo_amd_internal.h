@@ -76,4 +76,18 @@ int build_layout(HostLayout& L, int dtype, int dims, int bucket, uint64_t n, con
                        (int)(dims));                                                \
   }
 
+// Float trees only (families whose reference requires a float axis).
+#define KD_DISPATCH_FK(dtype, dims, ...)                                            \
+  switch ((dtype) * 8 + (dims)) {                                                   \
+    case KD_F32 * 8 + 2: { using A = float; constexpr int KK = 2; __VA_ARGS__; break; }    \
+    case KD_F32 * 8 + 3: { using A = float; constexpr int KK = 3; __VA_ARGS__; break; }    \
+    case KD_F32 * 8 + 4: { using A = float; constexpr int KK = 4; __VA_ARGS__; break; }    \
+    case KD_F64 * 8 + 2: { using A = double; constexpr int KK = 2; __VA_ARGS__; break; }   \
+    case KD_F64 * 8 + 3: { using A = double; constexpr int KK = 3; __VA_ARGS__; break; }   \
+    case KD_F64 * 8 + 4: { using A = double; constexpr int KK = 4; __VA_ARGS__; break; }   \
+    default:                                                                        \
+      return ::kd::set_error(KD_ERR_UNSUPPORTED, "this family needs a float tree (dtype %d / dims %d)", \
+                             (int)(dtype), (int)(dims));                            \
+  }
+
 }  // namespace kd

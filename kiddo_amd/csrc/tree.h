@@ -104,5 +104,12 @@ int launch_best_n_within(const QueryArgs& a, int n, double radius, uint32_t* ite
 int launch_within_count(const QueryArgs& a, double radius, uint64_t* counts);
 int launch_within_fill(const QueryArgs& a, double radius, const uint64_t* offsets, uint32_t* items,
                        void* dists);
+// k_periodic.hip (box: K lengths of the tree's coordinate type, host memory)
+int launch_periodic_nearest_one(const QueryArgs& a, const void* box, uint32_t* items, void* dists);
+int launch_periodic_nearest_n(const QueryArgs& a, const void* box, int k, double radius, uint32_t* items,
+                              void* dists, uint32_t* counts);
+int launch_periodic_within_count(const QueryArgs& a, const void* box, double radius, uint64_t* counts);
+int launch_periodic_within_fill(const QueryArgs& a, const void* box, double radius, const uint64_t* offsets,
+                                uint32_t* items, void* dists);
 
 }  // namespace kd

@@ -48,8 +48,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -260,6 +262,7 @@ template <class A, int K> struct View {
   const uint8_t* arena;
   uint64_t n;
   int max_stem_level;
+  const O* pbox = nullptr;  // periodic box (widened); null: Cartesian
 
   // src/leaf_view/mod.rs:431-446 tile walk; calls f(O dist, uint32_t item) per entry
   // with dist computed as ((0 + dist1(x)) + dist1(y)) + ... in O, no FMA.
@@ -274,7 +277,16 @@ template <class A, int K> struct View {
         for (int d = 0; d < K; ++d) {
           A c;
           std::memcpy(&c, p + (size_t(d) * tl + i) * sizeof(A), sizeof(A));
-          O diff = O(c) - qw[d];
+          O diff;
+          if (pbox) {
+            // minimum-image axis delta (query/periodic.rs:282-292, :322-336):
+            // delta = |q - c|, wrapped = L - delta, the smaller of the two
+            O delta = std::fabs(qw[d] - O(c));
+            O wrapped = pbox[d] - delta;
+            diff = wrapped < delta ? wrapped : delta;
+          } else {
+            diff = O(c) - qw[d];
+          }
           acc += diff * diff;
         }
         uint32_t it;
@@ -391,6 +403,103 @@ template <class A, int K> struct View {
         break;
       }
       if (!entered) break;
+    }
+  }
+
+  // Periodic backtracking (query/periodic.rs:439-618). Every frame carries the node's box
+  // (lower/upper per axis, starting at [0, L]), its per-axis periodic interval offsets and
+  // rect distance. At a stem both children get rd = (rd - off[dim]^2) + child_off^2 with
+  // child_off the distance from the query to the child's interval or its +-L images
+  // (:294-317); the far child is pushed unless pruned, a pruned near child pops the stack,
+  // and a popped frame is not re-checked. Requires pbox.
+  static O iv1(O q, O lo, O hi) {  // interval_distance_1d, stem_strategy/donnelly/simd_full/mod.rs:209-216
+    O below = std::max(O(0), lo - q);
+    O above = std::max(O(0), q - hi);
+    return below + above;
+  }
+  static O piv(O q, O lo, O hi, O L) {  // periodic_interval_distance_1d
+    O best = iv1(q, lo, hi);
+    O minus = iv1(q, lo - L, hi - L);
+    if (minus < best) best = minus;
+    O plus = iv1(q, lo + L, hi + L);
+    if (plus < best) best = plus;
+    return best;
+  }
+  template <class Ctx> void periodic_dfs(const A* q, Ctx& ctx, Stats& st) const {
+    if (n == 0) return;
+    struct Frame {
+      uint64_t idx;
+      int level;
+      O lower[K], upper[K], off[K];
+      O rd;
+    };
+    O qw[K];
+    Frame cur;
+    for (int d = 0; d < K; ++d) {
+      qw[d] = O(q[d]);
+      cur.lower[d] = O(0);
+      cur.upper[d] = pbox[d];
+      cur.off[d] = O(0);
+    }
+    cur.rd = O(0);
+    cur.idx = 1;
+    cur.level = 0;
+    std::vector<Frame> stack;
+    stack.reserve(size_t(max_stem_level) + 2);
+    for (;;) {
+      for (;;) {  // descend until a leaf is scanned (periodic.rs:495-606)
+        if (cur.level > max_stem_level) {
+          uint64_t leaf = cur.idx & ~(uint64_t(1) << cur.level);
+          ++st.leaves;
+          ctx.leaf(leaf, qw, st);
+          break;
+        }
+        const int dim = cur.level % K;
+        A pivot = stems[cur.idx];
+        ++st.stems;
+        if (!(pivot < Ax<A>::maxv())) {  // sentinel: left child, nothing changes
+          cur.idx *= 2;
+          ++cur.level;
+          continue;
+        }
+        const O pw = O(pivot), ol = cur.lower[dim], ou = cur.upper[dim];
+        const bool right = q[dim] >= pivot;
+        const O r_lo = std::max(ol, pw), r_hi = ou;
+        const O l_lo = ol, l_hi = ou < pw ? ou : pw;
+        const O n_lo = right ? r_lo : l_lo, n_hi = right ? r_hi : l_hi;
+        const O f_lo = right ? l_lo : r_lo, f_hi = right ? l_hi : r_hi;
+        const O n_off = piv(qw[dim], n_lo, n_hi, pbox[dim]);
+        const O f_off = piv(qw[dim], f_lo, f_hi, pbox[dim]);
+        const O old = cur.off[dim];
+        const O n_rd = (cur.rd - old * old) + n_off * n_off;
+        const O f_rd = (cur.rd - old * old) + f_off * f_off;
+        const bool n_pruned = ctx.prune(n_rd), f_pruned = ctx.prune(f_rd);
+        if (!f_pruned) {
+          Frame f = cur;
+          f.idx = cur.idx * 2 + (right ? 0 : 1);
+          f.level = cur.level + 1;
+          f.lower[dim] = f_lo;
+          f.upper[dim] = f_hi;
+          f.off[dim] = f_off;
+          f.rd = f_rd;
+          stack.push_back(f);
+        }
+        if (n_pruned) {
+          if (stack.empty()) return;
+          cur = stack.back();
+          stack.pop_back();
+          continue;
+        }
+        cur.idx = cur.idx * 2 + (right ? 1 : 0);
+        ++cur.level;
+        cur.lower[dim] = n_lo;
+        cur.upper[dim] = n_hi;
+        cur.off[dim] = n_off;
+        cur.rd = n_rd;
+      }
+      if (stack.empty()) break;
+      cur = stack.back();
+      stack.pop_back();
     }
   }
 };
@@ -821,6 +930,200 @@ int or_best_n_within(const or_tree* t, const void* q, uint64_t m, double radius,
       }
     });
     acc.total(stats3);
+    return 0;
+  });
+}
+
+// ------------------------------------------------------------ periodic ------------
+// Periodic boundary conditions (query/periodic.rs, builder.rs:1280-1291): float trees only
+// (PeriodicAxis is implemented for f32/f64), every box length strictly positive
+// (builder.rs:29-35). box: K values of the tree's coordinate type.
+
+}  // extern "C"
+
+#define DISPATCH_FLOAT(dtype, K, ...)                                        \
+  switch ((dtype) * 8 + (K)) {                                                \
+    case DT_F32 * 8 + 2: { using A = float; constexpr int KK = 2; __VA_ARGS__; }     \
+    case DT_F32 * 8 + 3: { using A = float; constexpr int KK = 3; __VA_ARGS__; }     \
+    case DT_F32 * 8 + 4: { using A = float; constexpr int KK = 4; __VA_ARGS__; }     \
+    case DT_F64 * 8 + 2: { using A = double; constexpr int KK = 2; __VA_ARGS__; }    \
+    case DT_F64 * 8 + 3: { using A = double; constexpr int KK = 3; __VA_ARGS__; }    \
+    case DT_F64 * 8 + 4: { using A = double; constexpr int KK = 4; __VA_ARGS__; }    \
+    default: return -3;                                                       \
+  }
+
+namespace {
+template <class A, int K> bool periodic_view(View<A, K>& v, const void* box, typename Ax<A>::O* bw) {
+  const A* b = static_cast<const A*>(box);
+  for (int d = 0; d < K; ++d) {
+    if (!(b[d] > A(0))) return false;
+    bw[d] = typename Ax<A>::O(b[d]);
+  }
+  v.pbox = bw;
+  return true;
+}
+}  // namespace
+
+extern "C" {
+
+// nearest_one under periodic boundaries (periodic.rs:196-244): Cartesian nearest_one of the
+// home query, then of each shifted image (every axis shifted by -L, 0 or +L, not all zero,
+// enumerated axis 0 outermost, shift -1, 0, +1, :47-142) whose lower bound
+// ((0 + o0^2) + o1^2) + ... (o = distance of the shifted coordinate outside [0, L], :27-37)
+// passes the best distance so far, visited in ascending lower bound (stable: ties in
+// enumeration order). Faithful: exclusive lower-bound checks and strict '<' updates, as
+// Kiddo. Canonical: inclusive checks and (distance, item) order, so equal-distance images
+// are seen.
+int or_periodic_nearest_one(const or_tree* t, const void* q, uint64_t m, const void* box, int mode,
+                            uint32_t* items, void* dists, int threads, uint64_t* stats3) {
+  if (t->L.dtype != DT_F32 && t->L.dtype != DT_F64) return -3;
+  StatsAcc acc(threads);
+  DISPATCH_FLOAT(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    O bw[KK];
+    if (!periodic_view(v, box, bw)) return -2;
+    auto cv = view_of<A, KK>(t);  // Cartesian view for the image searches
+    const A* qs = static_cast<const A*>(q);
+    const A* bx = static_cast<const A*>(box);
+    O* od = static_cast<O*>(dists);
+    const bool canonical = mode == 0;
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      const A* qi = qs + i * KK;
+      NearestOneCtx<A, KK> home{cv, canonical};
+      cv.dfs(qi, home, acc.per[th]);
+      O best = home.best;
+      uint32_t item = home.item;
+      struct Img { A wq[KK]; O lb; };
+      std::vector<Img> cands;
+      A wq[KK];
+      for (int d = 0; d < KK; ++d) wq[d] = qi[d];
+      auto accepts = [&](O lb) { return canonical ? !(lb > best) : lb < best; };
+      // recursion of periodic_image_candidates (:55-124), threshold = home best
+      std::function<void(int, bool, O)> rec = [&](int axis, bool shifted, O lb) {
+        if (axis == KK) {
+          if (shifted) {
+            Img c;
+            std::memcpy(c.wq, wq, sizeof(wq));
+            c.lb = lb;
+            cands.push_back(c);
+          }
+          return;
+        }
+        const A orig = qi[axis], L = bx[axis];
+        for (int sh = -1; sh <= 1; ++sh) {
+          A wc = sh < 0 ? A(orig - L) : sh == 0 ? orig : A(orig + L);
+          wq[axis] = wc;
+          A off = wc < A(0) ? A(std::fabs(wc - A(0))) : wc > L ? A(std::fabs(wc - L)) : A(0);
+          O ow = O(off);
+          O nlb = lb + ow * ow;
+          if (!accepts(nlb)) continue;
+          rec(axis + 1, shifted || sh != 0, nlb);
+        }
+        wq[axis] = orig;
+      };
+      rec(0, false, O(0));
+      std::stable_sort(cands.begin(), cands.end(), [](const Img& a, const Img& b) { return a.lb < b.lb; });
+      for (const Img& c : cands) {
+        if (!accepts(c.lb)) continue;
+        NearestOneCtx<A, KK> img{cv, canonical};
+        cv.dfs(c.wq, img, acc.per[th]);
+        const bool better = canonical ? (img.best < best || (img.best == best && img.item < item))
+                                      : img.best < best;
+        if (better) {
+          best = img.best;
+          item = img.item;
+        }
+      }
+      items[i] = item;
+      od[i] = best;
+    });
+    acc.total(stats3);
+    return 0;
+  });
+}
+
+// periodic nearest_n (radius +inf) and nearest_n().within(radius) (periodic.rs:620-791).
+int or_periodic_nearest_n(const or_tree* t, const void* q, uint64_t m, const void* box, uint64_t k,
+                          double radius, int exclusive, int sorted, int mode, uint32_t* items,
+                          void* dists, uint32_t* counts, int threads, uint64_t* stats3) {
+  if (k == 0) return -1;
+  if (t->L.dtype != DT_F32 && t->L.dtype != DT_F64) return -3;
+  StatsAcc acc(threads);
+  DISPATCH_FLOAT(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    O bw[KK];
+    if (!periodic_view(v, box, bw)) return -2;
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      NearestNCtx<A, KK> c{v, mode == 0, k, O(radius), exclusive != 0, {}};
+      c.v.reserve(k + 1);
+      v.periodic_dfs(qs + i * KK, c, acc.per[th]);
+      auto& r = c.v;
+      if (mode != 0 && sorted && r.size() < k)
+        std::stable_sort(r.begin(), r.end(), [](const Res<O>& a, const Res<O>& b) { return a.d < b.d; });
+      counts[i] = (uint32_t)r.size();
+      for (uint64_t j = 0; j < k; ++j) {
+        items[i * k + j] = j < r.size() ? r[j].it : 0xFFFFFFFFu;
+        od[i * k + j] = j < r.size() ? r[j].d : out_max<O>();
+      }
+    });
+    acc.total(stats3);
+    return 0;
+  });
+}
+
+int or_periodic_within_count(const or_tree* t, const void* q, uint64_t m, const void* box,
+                             double radius, int exclusive, uint64_t* counts, int threads,
+                             uint64_t* stats3) {
+  if (t->L.dtype != DT_F32 && t->L.dtype != DT_F64) return -3;
+  StatsAcc acc(threads);
+  DISPATCH_FLOAT(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    O bw[KK];
+    if (!periodic_view(v, box, bw)) return -2;
+    const A* qs = static_cast<const A*>(q);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      WithinCtx<A, KK> c{v, O(radius), exclusive != 0, nullptr};
+      v.periodic_dfs(qs + i * KK, c, acc.per[th]);
+      counts[i] = c.count;
+    });
+    acc.total(stats3);
+    return 0;
+  });
+}
+
+int or_periodic_within_fill(const or_tree* t, const void* q, uint64_t m, const void* box,
+                            double radius, int exclusive, int sorted, int mode,
+                            const uint64_t* offsets, uint32_t* items, void* dists, int threads) {
+  if (t->L.dtype != DT_F32 && t->L.dtype != DT_F64) return -3;
+  StatsAcc acc(threads);
+  DISPATCH_FLOAT(t->L.dtype, t->L.K, {
+    using O = typename Ax<A>::O;
+    auto v = view_of<A, KK>(t);
+    O bw[KK];
+    if (!periodic_view(v, box, bw)) return -2;
+    const A* qs = static_cast<const A*>(q);
+    O* od = static_cast<O*>(dists);
+    parallel_for(m, threads, [&](uint64_t i, int th) {
+      std::vector<Res<O>> r;
+      WithinCtx<A, KK> c{v, O(radius), exclusive != 0, &r};
+      v.periodic_dfs(qs + i * KK, c, acc.per[th]);
+      if (sorted) {
+        if (mode == 0)
+          std::sort(r.begin(), r.end(), canon_less<O>);
+        else
+          std::stable_sort(r.begin(), r.end(), [](const Res<O>& a, const Res<O>& b) { return a.d < b.d; });
+      }
+      uint64_t o = offsets[i];
+      for (size_t j = 0; j < r.size() && o + j < offsets[i + 1]; ++j) {
+        items[o + j] = r[j].it;
+        od[o + j] = r[j].d;
+      }
+    });
     return 0;
   });
 }

@@ -62,6 +62,11 @@ def lib():
         L.or_within_count.argtypes = [vp, vp, u64, f64, i32, vp, i32, vp]
         L.or_within_fill.argtypes = [vp, vp, u64, f64, i32, i32, i32, vp, vp, vp, i32]
         L.or_best_n_within.argtypes = [vp, vp, u64, f64, u64, i32, vp, vp, vp, i32, vp]
+        L.or_periodic_nearest_one.argtypes = [vp, vp, u64, vp, i32, vp, vp, i32, vp]
+        L.or_periodic_nearest_n.argtypes = [vp, vp, u64, vp, u64, f64, i32, i32, i32, vp, vp, vp,
+                                            i32, vp]
+        L.or_periodic_within_count.argtypes = [vp, vp, u64, vp, f64, i32, vp, i32, vp]
+        L.or_periodic_within_fill.argtypes = [vp, vp, u64, vp, f64, i32, i32, i32, vp, vp, vp, i32]
         L.or_brute_nearest_n.argtypes = [vp, vp, u64, u64, f64, i32, vp, vp, vp, i32]
         L.or_brute_within.argtypes = [vp, vp, u64, f64, i32, vp, vp, vp, vp, i32]
         L.or_brute_best_n_within.argtypes = [vp, vp, u64, f64, u64, i32, vp, vp, vp, i32]
@@ -231,6 +236,55 @@ class Tree:
         _check(lib().or_within_fill(self._h, _p(q), m, float(radius), int(exclusive),
                                     int(sorted), mode, _p(offsets), _p(items), _p(d), threads),
                "within_fill")
+        return (offsets, items, d, st) if stats else (offsets, items, d)
+
+    # -- periodic boundary conditions (float trees; box: K positive lengths) -------------
+    def _box(self, box):
+        b = np.ascontiguousarray(box, dtype=A_DTYPE[self.dtype_code])
+        assert b.shape == (self.K,)
+        return b
+
+    def periodic_nearest_one(self, q, box, mode=CANONICAL, threads=1, stats=False):
+        q, b = self._q(q), self._box(box)
+        m = q.shape[0]
+        items = np.zeros(m, np.uint32)
+        d = self._out(m)
+        st = np.zeros(3, np.uint64)
+        _check(lib().or_periodic_nearest_one(self._h, _p(q), m, _p(b), mode, _p(items), _p(d),
+                                             threads, _p(st)), "periodic_nearest_one")
+        return (items, d, st) if stats else (items, d)
+
+    def periodic_nearest_n(self, q, box, k, radius=np.inf, exclusive=False, sorted=True,
+                           mode=CANONICAL, threads=1, stats=False):
+        q, b = self._q(q), self._box(box)
+        m = q.shape[0]
+        items = np.zeros((m, k), np.uint32)
+        d = self._out(m, k)
+        counts = np.zeros(m, np.uint32)
+        st = np.zeros(3, np.uint64)
+        _check(lib().or_periodic_nearest_n(self._h, _p(q), m, _p(b), k, float(radius),
+                                           int(exclusive), int(sorted), mode, _p(items), _p(d),
+                                           _p(counts), threads, _p(st)), "periodic_nearest_n")
+        return (items, d, counts, st) if stats else (items, d, counts)
+
+    def periodic_within(self, q, box, radius, exclusive=False, sorted=True, mode=CANONICAL,
+                        threads=1, stats=False):
+        """Returns CSR (offsets[m+1], items, dists)."""
+        q, b = self._q(q), self._box(box)
+        m = q.shape[0]
+        counts = np.zeros(m, np.uint64)
+        st = np.zeros(3, np.uint64)
+        _check(lib().or_periodic_within_count(self._h, _p(q), m, _p(b), float(radius),
+                                              int(exclusive), _p(counts), threads, _p(st)),
+               "periodic_within_count")
+        offsets = np.zeros(m + 1, np.uint64)
+        np.cumsum(counts, out=offsets[1:])
+        total = int(offsets[-1])
+        items = np.zeros(total, np.uint32)
+        d = self._out(total)
+        _check(lib().or_periodic_within_fill(self._h, _p(q), m, _p(b), float(radius),
+                                             int(exclusive), int(sorted), mode, _p(offsets),
+                                             _p(items), _p(d), threads), "periodic_within_fill")
         return (offsets, items, d, st) if stats else (offsets, items, d)
 
     def best_n_within(self, q, radius, n, exclusive=False, threads=1, stats=False):

@@ -41,6 +41,15 @@ class OracleEngine:
     def best_n_within(self, t, q, radius, n, exclusive=False):
         return t.best_n_within(q, radius, n, exclusive)
 
+    def periodic_nearest_one(self, t, q, box):
+        return t.periodic_nearest_one(q, box)
+
+    def periodic_nearest_n(self, t, q, box, k, radius=math.inf, exclusive=False, sorted=True):
+        return t.periodic_nearest_n(q, box, k, radius, exclusive, sorted)
+
+    def periodic_within(self, t, q, box, radius, exclusive=False, sorted=True):
+        return t.periodic_within(q, box, radius, exclusive, sorted)
+
 
 class GpuEngine:
     name = "gpu"
@@ -87,6 +96,25 @@ class GpuEngine:
         r = t.best_n_within_raw(np.asarray(q, t.a_dtype), radius, n, f)
         return self._host(r.items), self._host(r.distances), self._host(r.counts)
 
+    def _pflags(self, exclusive=False, sorted=True):
+        f = (self.flags & ~self.ka.KD_SCREEN_F32) | (self.ka.KD_EXCLUSIVE if exclusive else 0)
+        return f if sorted else f | self.ka.KD_UNSORTED
+
+    def periodic_nearest_one(self, t, q, box):
+        r = t.periodic_nearest_one_raw(np.asarray(q, t.a_dtype), box, self._pflags())
+        return self._host(r.item), self._host(r.distance)
+
+    def periodic_nearest_n(self, t, q, box, k, radius=math.inf, exclusive=False, sorted=True):
+        r = t.periodic_nearest_n_raw(np.asarray(q, t.a_dtype), box, k, radius,
+                                     self._pflags(exclusive, sorted))
+        return self._host(r.items), self._host(r.distances), self._host(r.counts)
+
+    def periodic_within(self, t, q, box, radius, exclusive=False, sorted=True):
+        r = t.periodic_within_raw(np.asarray(q, t.a_dtype), box, radius,
+                                  self._pflags(exclusive, sorted))
+        return (self._host(r.offsets).astype(np.uint64), self._host(r.items),
+                self._host(r.distances))
+
 
 def run_kat_case(engine, case):
     """Runs one reference known-answer case; asserts on mismatch."""
@@ -97,6 +125,9 @@ def run_kat_case(engine, case):
     for qc in case["queries"]:
         q = np.array([qc["q"]], dtype=dt)
         fam = qc["family"]
+        if "box" in qc:
+            _run_periodic_kat(engine, t, case, qc, q, np.array(qc["box"], dtype=dt))
+            continue
         if fam == "nearest_one":
             it, d = engine.nearest_one(t, q)
             assert int(it[0]) == qc["item"], (case["name"], qc, it)
@@ -129,6 +160,46 @@ def run_kat_case(engine, case):
             assert got == qc["items"], (case["name"], qc, got)
         else:
             raise AssertionError(f"unknown family {fam}")
+
+
+def _run_periodic_kat(engine, t, case, qc, q, box):
+    """Known answers under periodic boundary conditions (query/builder.rs tests)."""
+    fam = qc["family"]
+    radius = qc.get("radius", math.inf)
+    if qc.get("radius_from_nearest_one"):
+        _, d = engine.periodic_nearest_one(t, q, box)
+        radius = float(d[0])
+    if fam == "nearest_one":
+        it, d = engine.periodic_nearest_one(t, q, box)
+        assert int(it[0]) == qc["item"], (case["name"], qc, it)
+        assert abs(float(d[0]) - qc["dist"]) < qc["dist_abs_tol"], (case["name"], float(d[0]))
+    elif fam == "nearest_n_within_unsorted":
+        it, d, c = engine.periodic_nearest_n(t, q, box, qc["k"], radius, qc.get("exclusive", False),
+                                             False)
+        got = sorted(int(x) for x in it[0][: int(c[0])])
+        assert got == qc["items_sorted"], (case["name"], qc, got)
+    elif fam == "within_unsorted":
+        off, it, d = engine.periodic_within(t, q, box, radius, qc.get("exclusive", False), False)
+        got = sorted(int(x) for x in it[int(off[0]): int(off[1])])
+        assert got == qc["items_sorted"], (case["name"], qc, got)
+    else:
+        raise AssertionError(f"unknown periodic family {fam}")
+
+
+def min_image_sqdist(points: np.ndarray, q: np.ndarray, box: np.ndarray, out_dtype) -> np.ndarray:
+    """Squared distance under periodic boundaries, evaluated independently with numpy in
+    the order Kiddo's periodic leaf scan uses: per axis delta = |q - p|, the smaller of
+    delta and L - delta, then acc = acc + m*m from 0, each op rounded."""
+    p = points.astype(out_dtype)
+    qq = q.astype(out_dtype)
+    L = box.astype(out_dtype)
+    acc = np.zeros(np.broadcast_shapes(p.shape[:-1], qq.shape[:-1]), out_dtype)
+    for d in range(p.shape[-1]):
+        delta = np.abs((qq[..., d] - p[..., d]).astype(out_dtype))
+        wrapped = (L[d] - delta).astype(out_dtype)
+        m = np.where(wrapped < delta, wrapped, delta).astype(out_dtype)
+        acc = (acc + (m * m).astype(out_dtype)).astype(out_dtype)
+    return acc
 
 
 def assert_same_bits(a, b, what=""):
