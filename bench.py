@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--no-within", action="store_true", help="knn16: skip the within(r=0.002) pass")
-    ap.add_argument("--workload", choices=["knn16", "forest", "serving", "mixture", "small"],
+    ap.add_argument("--workload", choices=["knn16", "forest", "serving", "mixture", "small", "periodic"],
                     default="knn16",
                     help="knn16: the headline 16M-point k-NN; forest: 4096-tree batched form; "
                          "serving: continuous batching over resident trees")
@@ -92,6 +92,8 @@ def main():
         return bench_mixture(args, torch, ka)
     if args.workload == "small":
         return bench_small(args, torch, ka)
+    if args.workload == "periodic":
+        return bench_periodic(args, torch, ka)
 
     if args.copy_probe:
         # A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it scales the
@@ -586,6 +588,78 @@ def bench_small(args, torch, ka):
         "data": "synthetic (seeded uniform)", "config": {"workload": "small-100k-f64", "points": n,
                                                          "queries": m},
         "families": res,
+    }), flush=True)
+
+
+def bench_periodic(args, torch, ka):
+    """Periodic boundary conditions (query/periodic.rs) on the unit box: 16M uniform f64
+    points, 16M uniform queries, periodic nearest_n(16), nearest_one and within(r = 0.002),
+    Morton-ordered, next to the Cartesian nearest_n(16) of the same tree and queries. CPU
+    baseline: the oracle's periodic nearest_n on a sample of the queries."""
+    n, m, k = args.points, args.queries, args.k
+    pts = np.random.default_rng(args.seed).random((n, 3))
+    qh = np.random.default_rng(args.seed + 1).random((m, 3))
+    box = np.ones(3)
+    tree = ka.ImmutableKdTree.new_from_slice(pts, args.bucket)
+    tree.set_timing(True)
+    q = torch.from_numpy(qh).cuda()
+    M = ka.KD_MORTON
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kms = []
+        for _ in range(reps):
+            fn()
+            kms.append(tree.last_kernel_ms())
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, float(np.mean(kms))
+
+    res = {}
+    dt, kms = timed(lambda: tree.periodic_nearest_n_raw(q, box, k, flags=M), args.steps)
+    st = tree.query_stats()
+    res[f"periodic_nearest_n{k}"] = {"ms": round(dt * 1e3, 2), "kernel_ms": round(kms, 2),
+                                    "qps": round(m / dt, 1), "leaves_per_query": round(st["leaves"] / m, 2)}
+    dtc, kmsc = timed(lambda: tree.nearest_n_raw(q, k, flags=M), args.steps)
+    res[f"cartesian_nearest_n{k}"] = {"ms": round(dtc * 1e3, 2), "kernel_ms": round(kmsc, 2),
+                                     "qps": round(m / dtc, 1)}
+    dt1, _ = timed(lambda: tree.periodic_nearest_one_raw(q, box, M), args.steps)
+    res["periodic_nearest_one"] = {"ms": round(dt1 * 1e3, 2), "qps": round(m / dt1, 1)}
+    t0 = time.perf_counter()
+    w = tree.periodic_within_raw(q, box, 0.002 ** 2, M)
+    torch.cuda.synchronize()
+    dtw = time.perf_counter() - t0
+    res["periodic_within_r0.002"] = {"ms": round(dtw * 1e3, 2), "results": int(w.offsets[-1].item())}
+    del w
+    log(f"periodic: {res}")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.build()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ot = O.Tree.build(pts, args.bucket)
+        ns = min(args.cpu_sample, m)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ot.periodic_nearest_n(qh[:ns], box, k, mode=O.FAITHFUL, threads=threads)
+            runs.append(time.perf_counter() - t0)
+        cdt = float(np.median(runs))
+        cpu = dict(value=round(ns / cdt, 1), unit="queries/s", cores=threads, kind="port",
+                   sample=f"first {ns} of the {m} queries, periodic nearest_n(k={k}), Kiddo's "
+                          f"periodic query restated in C++ (oracle/kiddo_oracle.cpp), {threads} "
+                          f"threads, median of 3 runs", seconds=round(cdt, 2))
+    print(json.dumps({
+        "metric": f"queries/s, {n // 1_000_000}M-pt f64 3D periodic (unit box) nearest_n({k})",
+        "value": res[f"periodic_nearest_n{k}"]["qps"], "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res[f"periodic_nearest_n{k}"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded uniform in the unit box)",
+        "config": {"workload": f"periodic-f64-3d-{n}-{m}q", "points": n, "queries": m, "k": k,
+                   "box": [1.0, 1.0, 1.0]},
+        "families": res, "cpu_baseline": cpu,
     }), flush=True)
 
 
