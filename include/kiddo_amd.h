@@ -53,6 +53,22 @@ enum kd_query_flags {
   KD_GRAPH = 1u << 4        /* forests: capture the launch in a hipGraph and replay it     */
 };
 
+/* Distance metric of a query (the `D` of `nearest_n::<D>(k)` etc., src/dist/): bits 8-11
+ * of the flags hold the kind, bits 12-19 the Minkowski exponent P. Every tree query entry
+ * point (Cartesian and periodic, all families) and the server take it; batched forests
+ * and KD_SCREEN_F32 are SquaredEuclidean only (KD_ERR_UNSUPPORTED). Radii are in the
+ * metric's own units (Manhattan and Chebyshev: plain distance; Minkowski: sum of |d|^P).
+ *   KD_METRIC_SQEUCLIDEAN  squared_euclidean/mod.rs: sum of d*d (the default, 0)
+ *   KD_METRIC_MANHATTAN    manhattan/mod.rs:36-49: sum of |d|
+ *   KD_METRIC_CHEBYSHEV    chebyshev/mod.rs:34-61: max of |d|
+ *   KD_METRIC_MINKOWSKI(P) minkowski/mod.rs:55-59: sum of |d|^P, P >= 3 (P = 1, 2: use the
+ *                          two above, as the reference requires; KD_ERR_INVALID) */
+#define KD_METRIC_SQEUCLIDEAN 0u
+#define KD_METRIC_MANHATTAN (1u << 8)
+#define KD_METRIC_CHEBYSHEV (2u << 8)
+#define KD_METRIC_MINKOWSKI(p) ((3u << 8) | ((uint32_t)(p) << 12))
+#define KD_METRIC_MASK 0x000FFF00u
+
 /* tree-create flags */
 enum kd_tree_flags {
   KD_TREE_SHADOW_F32 = 1u << 0 /* f64 trees: keep an FP32 shadow of the leaves            */

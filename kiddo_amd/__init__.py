@@ -37,6 +37,27 @@ KD_MORTON = 1 << 2
 KD_SCREEN_F32 = 1 << 3
 KD_GRAPH = 1 << 4
 KD_TREE_SHADOW_F32 = 1 << 0
+# distance metrics (flag bits 8-19, include/kiddo_amd.h KD_METRIC_*)
+KD_METRIC_SQEUCLIDEAN = 0
+KD_METRIC_MANHATTAN = 1 << 8
+KD_METRIC_CHEBYSHEV = 2 << 8
+KD_METRIC_MASK = 0x000FFF00
+
+
+def KD_METRIC_MINKOWSKI(p: int) -> int:
+    return (3 << 8) | (int(p) << 12)
+
+
+def metric_flags(metric) -> int:
+    """'sqeuclidean' | 'manhattan' | 'chebyshev' | ('minkowski', P) | flag value -> flag bits."""
+    if metric is None:
+        return 0
+    if isinstance(metric, int):
+        return metric
+    if isinstance(metric, tuple) and metric[0] == "minkowski":
+        return KD_METRIC_MINKOWSKI(metric[1])
+    return {"sqeuclidean": 0, "squared_euclidean": 0, "manhattan": KD_METRIC_MANHATTAN,
+            "chebyshev": KD_METRIC_CHEBYSHEV}[metric]
 
 _NP_CODE = {np.dtype(np.float32): KD_F32, np.dtype(np.float64): KD_F64,
             np.dtype(np.uint16): KD_U16, np.dtype(np.uint32): KD_U32}
@@ -648,23 +669,27 @@ class QueryBuilder:
         return self
 
     # families (builder.rs:2008-2057)
-    def nearest_one(self):
+    def nearest_one(self, metric=None):
+        self.flags |= metric_flags(metric)
         return self._family("nearest_one")
 
-    def nearest_n(self, k: int):
+    def nearest_n(self, k: int, metric=None):
         if k < 1:
             raise ValueError("max_qty must be non-zero")
         self.k = k
+        self.flags |= metric_flags(metric)
         return self._family("nearest_n")
 
-    def within(self, radius: float):
+    def within(self, radius: float, metric=None):
         if self.family == "nearest_n":  # nearest_n(k).within(radius)
             self.radius = radius
             return self
         self.radius = radius
+        self.flags |= metric_flags(metric)
         return self._family("within")
 
-    def best_n_within(self, radius: float, n: int):
+    def best_n_within(self, radius: float, n: int, metric=None):
+        self.flags |= metric_flags(metric)
         if self.box is not None:
             raise ValueError("best_n_within is Cartesian-only (builder.rs:1287-1290)")
         if n < 1:
@@ -729,4 +754,5 @@ class QueryBuilder:
 __all__ = ["ImmutableKdTree", "HostLayout", "Forest", "Server", "KD_GRAPH", "FAMILY_NEAREST_ONE",
            "FAMILY_NEAREST_N", "FAMILY_BEST_N_WITHIN", "FAMILY_APPROX", "QueryBuilder", "NearestOne", "NearestN",
            "Within", "KiddoAmdError", "lib", "SIGNATURES", "KD_EXCLUSIVE", "KD_UNSORTED",
-           "KD_MORTON", "KD_SCREEN_F32", "KD_TREE_SHADOW_F32"]
+           "KD_MORTON", "KD_SCREEN_F32", "KD_TREE_SHADOW_F32", "KD_METRIC_SQEUCLIDEAN",
+           "KD_METRIC_MANHATTAN", "KD_METRIC_CHEBYSHEV", "KD_METRIC_MINKOWSKI", "metric_flags"]

@@ -414,7 +414,26 @@ int kd_tree_tune2(kd_tree* t, int32_t knn_variant, int32_t group_size, int32_t l
   return KD_OK;
 }
 
+namespace kd {
+namespace {
+// Metric field of the flags (KD_METRIC_*): kind 0..3; Minkowski exponents 1 and 2 are
+// refused as in minkowski/mod.rs:36-45 (use Manhattan / SquaredEuclidean); the FP32 screen
+// is derived for SquaredEuclidean only.
+int check_metric(uint32_t flags) {
+  const int kind = int((flags >> 8) & 0xFu), p = int((flags >> 12) & 0xFFu);
+  if (kind > 3) return set_error(KD_ERR_INVALID, "unknown metric %d", kind);
+  if (kind != 3 && p != 0) return set_error(KD_ERR_INVALID, "a Minkowski exponent needs KD_METRIC_MINKOWSKI");
+  if (kind == 3 && (p == 1 || p == 2))
+    return set_error(KD_ERR_INVALID, "Minkowski<%d>: use Manhattan (P = 1) or SquaredEuclidean (P = 2)", p);
+  if (kind != 0 && (flags & KD_SCREEN_F32))
+    return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 applies to SquaredEuclidean only");
+  return KD_OK;
+}
+}  // namespace
+}  // namespace kd
+
 #define KD_ORDERED(a, queries_, m_, flags_, stream_)                                  \
+  if (int mrc_ = kd::check_metric(flags_)) return mrc_;                               \
   hipStream_t s_ = static_cast<hipStream_t>(stream_);                                 \
   kd::Order order_;                                                                   \
   {                                                                                   \

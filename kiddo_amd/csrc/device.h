@@ -78,6 +78,78 @@ template <class T, int K> struct Small {
 
 constexpr int kMaxLevels = 32;
 
+// ------------------------------------------------------------------- metrics ---------
+// Distance metrics (kiddo src/dist/*/mod.rs). Each gives dist1 of a coordinate difference
+// (or of an axis offset, i.e. dist1(x, 0)), how per-axis terms combine from 0 in axis
+// order (distance_metric_core.rs:38-40 and overrides), and the rect distance after one
+// axis offset changes (distance_metric_core.rs:89-98 and overrides). Every product and sum
+// is rounded separately, as in Kiddo's scalar path.
+struct SqEuclidean {  // squared_euclidean/mod.rs:36-39
+  template <class O> __device__ __forceinline__ O d1(O x) const { return mul_rn(x, x); }
+  template <class O> __device__ __forceinline__ O combine(O acc, O c) const { return add_rn(acc, c); }
+  template <class O, int K>
+  __device__ __forceinline__ O rect(O rd, const Small<O, K>& off, int dim, O no) const {
+    const O oo = off[dim];
+    return add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+  }
+};
+struct Manhattan {  // manhattan/mod.rs:36-49: |d|, sum, (rd - old) + new
+  template <class O> __device__ __forceinline__ O d1(O x) const { return fabs(x); }
+  template <class O> __device__ __forceinline__ O combine(O acc, O c) const { return add_rn(acc, c); }
+  template <class O, int K>
+  __device__ __forceinline__ O rect(O rd, const Small<O, K>& off, int dim, O no) const {
+    return add_rn(sub_rn(rd, off[dim]), no);
+  }
+};
+struct Chebyshev {  // chebyshev/mod.rs:34-61: |d|, max, offsets rescanned
+  template <class O> __device__ __forceinline__ O d1(O x) const { return fabs(x); }
+  template <class O> __device__ __forceinline__ O combine(O acc, O c) const { return c > acc ? c : acc; }
+  template <class O, int K>
+  __device__ __forceinline__ O rect(O, const Small<O, K>& off, int dim, O no) const {
+    O acc = O(0);
+#pragma unroll
+    for (int a = 0; a < K; ++a) acc = combine(acc, a == dim ? no : off[a]);
+    return acc;
+  }
+};
+struct Minkowski {  // minkowski/mod.rs:55-59: |d|^P, sum, (rd - old^P) + new^P
+  int p;
+  // powi by binary exponentiation (the expansion of llvm.powi, as compiler-rt's __powidf2)
+  template <class O> __device__ __forceinline__ O d1(O x) const {
+    O a = fabs(x), r = O(1);
+    for (int b = p;;) {
+      if (b & 1) r = mul_rn(r, a);
+      b >>= 1;
+      if (!b) break;
+      a = mul_rn(a, a);
+    }
+    return r;
+  }
+  template <class O> __device__ __forceinline__ O combine(O acc, O c) const { return add_rn(acc, c); }
+  template <class O, int K>
+  __device__ __forceinline__ O rect(O rd, const Small<O, K>& off, int dim, O no) const {
+    return add_rn(sub_rn(rd, d1(off[dim])), d1(no));
+  }
+};
+
+// A widened query that carries a metric other than SquaredEuclidean (a plain Small<O, K>
+// means SquaredEuclidean).
+template <class O, int K, class M> struct MetricQW {
+  Small<O, K> q;
+  M met;
+  __device__ __forceinline__ O operator[](int d) const { return q[d]; }
+};
+template <class QW> __device__ __forceinline__ SqEuclidean metric_of(const QW&) { return SqEuclidean{}; }
+template <class O, int K, class M> __device__ __forceinline__ M metric_of(const MetricQW<O, K, M>& w) {
+  return w.met;
+}
+template <class O, int K> __device__ __forceinline__ Small<O, K> wrap_qw(const Small<O, K>& qw, SqEuclidean) {
+  return qw;
+}
+template <class O, int K, class M> __device__ __forceinline__ MetricQW<O, K, M> wrap_qw(const Small<O, K>& qw, M met) {
+  return MetricQW<O, K, M>{qw, met};
+}
+
 template <class A, int K> struct DevTree {
   const A* __restrict__ stems;
   const uint32_t* __restrict__ leaf_start;
@@ -120,6 +192,18 @@ __device__ __forceinline__ typename Ax<A>::O point_dist(const DevTree<A, K>& t, 
   return acc;
 }
 
+// Distance of entry p under a metric other than SquaredEuclidean.
+template <class A, int K, class O, class M>
+__device__ __forceinline__ O point_dist(const DevTree<A, K>& t, uint32_t p, const MetricQW<O, K, M>& w) {
+  O acc = O(0);
+#pragma unroll
+  for (int d = 0; d < K; ++d) {
+    const O c = O(t.coords[d * t.plane + p]);
+    acc = w.met.combine(acc, w.met.d1(sub_rn(c, w.q[d])));
+  }
+  return acc;
+}
+
 // Per-thread traversal counters, summed per wave into the tree's counters: leaves
 // visited, entries scanned, stem nodes read, entries re-measured in FP64 (screen only).
 struct Counters {
@@ -128,9 +212,9 @@ struct Counters {
 
 // ------------------------------------------------------------------ traversal --------
 // V supplies bound(), prune(rd), unbounded() and leaf(tree, qw, leaf_index).
-template <class A, int K, class V>
-__device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, K>& q,
-                                         const Small<typename Ax<A>::O, K>& qw, V& vis, Counters& c) {
+template <class A, int K, class V, class QW>
+__device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, K>& q, const QW& qw, V& vis,
+                                         Counters& c) {
   // Register-only form of Kiddo's DFS: no per-level stack. A queued far child at level l is
   // a `pending` bit; its rect distance rd' = (rd - off[dim]^2) + new_off^2 is recomputed
   // when the bit is popped, from the same rd and off[] it was computed from at push time
@@ -140,6 +224,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
   // remaining taken levels top-down, with the exact operations used when they were entered.
   using O = typename Ax<A>::O;
   if (t.n == 0) return;  // orchestrator/mod.rs:399-401
+  const auto met = metric_of(qw);
   const int D = t.nlev;
   Small<O, K> off;
   off.fill(O(0));
@@ -159,8 +244,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
           pending |= 1u << level;
         } else {
           const O no = abs_diff(qw[dim], O(pivot));
-          const O oo = off[dim];
-          const O rdf = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+          const O rdf = met.rect(rd, off, dim, no);
           if (!(rdf > vis.bound())) pending |= 1u << level;
         }
         idx = (idx << 1) | (right ? 1u : 0u);
@@ -194,8 +278,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
           const A pv = stem_at(t, idx >> (D - tl));
           c.stems += 1;
           const O no = abs_diff(qw[td], O(pv));
-          const O oo = off[td];
-          rd = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+          rd = met.rect(rd, off, td, no);
           off.set(td, no);
         }
         continue;
@@ -205,8 +288,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
       const A pivot = stem_at(t, idx >> (D - l));
       c.stems += 1;
       const O no = abs_diff(qw[ld], O(pivot));
-      const O oo = off[ld];
-      const O frd = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(no, no));
+      const O frd = met.rect(rd, off, ld, no);
       if (vis.prune(frd)) continue;
       taken |= bit;
       off.set(ld, no);

@@ -1,0 +1,155 @@
+// families.h — the thread-per-query kernels of every query family, templated on the
+// distance metric M (device.h: SqEuclidean, Manhattan, Chebyshev, Minkowski).
+//   nearest_one      kiddo query/nearest_one.rs:111-143, :321-352
+//   approx           query/approx_nearest_one.rs:49-71 (descent to one leaf)
+//   nearest_n        query/nearest_n.rs:21-38, nearest_n_within.rs:127-179, :441-479
+//   best_n_within    query/best_n_within.rs:163-196, :290-306
+//   within           query/within.rs:18-41, within_unsorted.rs:133-156, :267-286
+// The metric is a kernel argument (Minkowski carries its exponent); a plain widened query
+// means SquaredEuclidean, so those instantiations are the original kernels.
+#pragma once
+
+#include "device.h"
+
+namespace kd {
+namespace dev {
+
+template <class A, int K, class M>
+__global__ __launch_bounds__(kBlock) void k_nearest_one(DevTree<A, K> t, const A* __restrict__ queries,
+                                                        const uint32_t* __restrict__ order, uint64_t m, M met,
+                                                        uint32_t* __restrict__ out_items,
+                                                        typename Ax<A>::O* __restrict__ out_d,
+                                                        unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    NearestOneVis<O> v{oinf<O>(), 0u};
+    traverse(t, q, mq, v, cnt);
+    out_items[qi] = v.item;
+    out_d[qi] = v.best;
+  }
+  flush_counters(cnt, stats);
+}
+
+template <class A, int K, class M>
+__global__ __launch_bounds__(kBlock) void k_approx_nearest_one(DevTree<A, K> t, const A* __restrict__ queries,
+                                                               const uint32_t* __restrict__ order, uint64_t m,
+                                                               M met, uint32_t* __restrict__ out_items,
+                                                               A* __restrict__ out_d,
+                                                               unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE  // O == A for float trees
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    NearestOneVis<O> v{oinf<O>(), 0u};
+    if (t.n > 0) {
+      // straight descent (orchestrator/mod.rs:133-151, traits/stem_strategy.rs:182-200)
+      uint32_t idx = 1;
+      int dim = 0;
+      for (int level = 0; level < t.nlev; ++level) {
+        const A pivot = stem_at(t, idx);
+        idx = (idx << 1) | (q[dim] >= pivot ? 1u : 0u);
+        dim = (dim + 1 == K) ? 0 : dim + 1;
+      }
+      const uint32_t leaf = idx - (1u << t.nlev);
+      cnt.stems += uint32_t(t.nlev);
+      cnt.leaves += 1;
+      cnt.points += t.leaf_start[leaf + 1] - t.leaf_start[leaf];
+      v.leaf(t, mq, leaf);
+    }
+    out_items[qi] = v.item;
+    out_d[qi] = A(v.best);
+  }
+  flush_counters(cnt, stats);
+}
+
+template <class A, int K, int MAXK, class M>
+__global__ __launch_bounds__(kBlock) void k_nearest_n(DevTree<A, K> t, const A* __restrict__ queries,
+                                                      const uint32_t* __restrict__ order, uint64_t m, M met,
+                                                      int k, typename Ax<A>::O radius, int excl,
+                                                      uint32_t* __restrict__ out_items,
+                                                      typename Ax<A>::O* __restrict__ out_d,
+                                                      uint32_t* __restrict__ out_counts,
+                                                      unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    KnnVis<O, MAXK> v;
+    v.init(k, radius, excl != 0);
+    traverse(t, q, mq, v, cnt);
+    uint32_t* ri = out_items + qi * uint64_t(k);
+    O* rdd = out_d + qi * uint64_t(k);
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j)
+      if (j < k) {
+        ri[j] = j < v.cnt ? v.top.it[j] : 0xFFFFFFFFu;
+        rdd[j] = j < v.cnt ? v.top.d[j] : oinf<O>();
+      }
+    out_counts[qi] = uint32_t(v.cnt);
+  }
+  flush_counters(cnt, stats);
+}
+
+template <class A, int K, int MAXK, class M>
+__global__ __launch_bounds__(kBlock) void k_best_n_within(DevTree<A, K> t, const A* __restrict__ queries,
+                                                          const uint32_t* __restrict__ order, uint64_t m, M met,
+                                                          int n, typename Ax<A>::O radius, int excl,
+                                                          uint32_t* __restrict__ out_items,
+                                                          typename Ax<A>::O* __restrict__ out_d,
+                                                          uint32_t* __restrict__ out_counts,
+                                                          unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    BestNVis<O, MAXK> v;
+    v.init(n, radius, excl != 0);
+    traverse(t, q, mq, v, cnt);
+    uint32_t* ri = out_items + qi * uint64_t(n);
+    O* rdd = out_d + qi * uint64_t(n);
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j)
+      if (j < n) {
+        ri[j] = j < v.cnt ? v.it[j] : 0xFFFFFFFFu;
+        rdd[j] = j < v.cnt ? v.d[j] : oinf<O>();
+      }
+    out_counts[qi] = uint32_t(v.cnt);
+  }
+  flush_counters(cnt, stats);
+}
+
+template <class A, int K, class M>
+__global__ __launch_bounds__(kBlock) void k_within_count(DevTree<A, K> t, const A* __restrict__ queries,
+                                                         const uint32_t* __restrict__ order, uint64_t m, M met,
+                                                         typename Ax<A>::O radius, int excl,
+                                                         uint64_t* __restrict__ counts,
+                                                         unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    WithinVis<O> v{radius, excl != 0, 0, nullptr, nullptr, 0};
+    traverse(t, q, mq, v, cnt);
+    counts[qi] = v.count;
+  }
+  flush_counters(cnt, stats);
+}
+
+template <class A, int K, class M>
+__global__ __launch_bounds__(kBlock) void k_within_fill(DevTree<A, K> t, const A* __restrict__ queries,
+                                                        const uint32_t* __restrict__ order, uint64_t m, M met,
+                                                        typename Ax<A>::O radius, int excl, int sorted,
+                                                        const uint64_t* __restrict__ offsets,
+                                                        uint32_t* __restrict__ out_items,
+                                                        typename Ax<A>::O* __restrict__ out_d,
+                                                        unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    const uint64_t o = offsets[qi], cap = offsets[qi + 1] - o;
+    WithinVis<O> v{radius, excl != 0, 0, out_items + o, out_d + o, cap};
+    traverse(t, q, mq, v, cnt);
+    if (sorted) sort_segment(out_items + o, out_d + o, v.count < cap ? v.count : cap);
+  }
+  flush_counters(cnt, stats);
+}
+
+}  // namespace dev
+}  // namespace kd

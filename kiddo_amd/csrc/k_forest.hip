@@ -699,6 +699,7 @@ int kd_forest_tune(kd_forest* f, int32_t variant, int32_t group_size, int32_t ld
 int kd_forest_nearest_n(kd_forest* f, const void* queries, uint32_t k, double radius, uint32_t flags,
                         uint32_t* items, void* dists, uint32_t* counts, void* stream) {
   if (!f) return kd::set_error(KD_ERR_INVALID, "forest is null");
+  if (flags & 0x000FFF00u) return kd::set_error(KD_ERR_UNSUPPORTED, "batched forests answer SquaredEuclidean only");
   if (k == 0 || k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_INVALID, "k must be in [1, %d]", kd::kMaxK);
   if (std::isnan(radius)) return kd::set_error(KD_ERR_INVALID, "radius is NaN");
   if (f->q_off.empty()) return kd::set_error(KD_ERR_INVALID, "call kd_forest_plan first");
@@ -753,6 +754,7 @@ int kd_forest_within(kd_forest* f, const void* queries, double radius, uint32_t 
                      uint32_t* items, void* dists, void* stream) {
   // offsets[m+1] must be filled by a count call (items == NULL) before the fill call.
   if (!f) return kd::set_error(KD_ERR_INVALID, "forest is null");
+  if (flags & 0x000FFF00u) return kd::set_error(KD_ERR_UNSUPPORTED, "batched forests answer SquaredEuclidean only");
   if (std::isnan(radius)) return kd::set_error(KD_ERR_INVALID, "radius is NaN");
   if (f->q_off.empty()) return kd::set_error(KD_ERR_INVALID, "call kd_forest_plan first");
   if (!offsets) return kd::set_error(KD_ERR_INVALID, "offsets is null");

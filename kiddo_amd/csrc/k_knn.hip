@@ -7,36 +7,10 @@
 //     top-k spread one slot per lane (device.h group_traverse / GroupKnnVis).
 // Each form has an FP32-screened variant for f64 trees (KD_SCREEN_F32).
 
-#include "tree.h"
+#include "metric_launch.h"
 
 namespace kd {
 namespace dev {
-
-template <class A, int K, int MAXK>
-__global__ __launch_bounds__(kBlock) void k_nearest_n(DevTree<A, K> t, const A* __restrict__ queries,
-                                                      const uint32_t* __restrict__ order, uint64_t m,
-                                                      int k, typename Ax<A>::O radius, int excl,
-                                                      uint32_t* __restrict__ out_items,
-                                                      typename Ax<A>::O* __restrict__ out_d,
-                                                      uint32_t* __restrict__ out_counts,
-                                                      unsigned long long* __restrict__ stats) {
-  KD_QUERY_PROLOGUE
-  if (active) {
-    KnnVis<O, MAXK> v;
-    v.init(k, radius, excl != 0);
-    traverse(t, q, qw, v, cnt);
-    uint32_t* ri = out_items + qi * uint64_t(k);
-    O* rdd = out_d + qi * uint64_t(k);
-#pragma unroll
-    for (int j = 0; j < MAXK; ++j)
-      if (j < k) {
-        ri[j] = j < v.cnt ? v.top.it[j] : 0xFFFFFFFFu;
-        rdd[j] = j < v.cnt ? v.top.d[j] : oinf<O>();
-      }
-    out_counts[qi] = uint32_t(v.cnt);
-  }
-  flush_counters(cnt, stats);
-}
 
 // Per-query FP32 screen set-up: the query rounded to FP32, and sqrt(K) times the bound on
 // the error of an FP32 coordinate difference (see screen_threshold).
@@ -162,8 +136,9 @@ int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, v
       return KD_OK;
     }
   }
-  hipLaunchKernelGGL((dev::k_nearest_n<A, K, MAXK>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds, a.s, dt,
-                     q, a.order, a.m, k, O(radius), excl, items, static_cast<O*>(dists), counts, t->counters);
+  hipLaunchKernelGGL((dev::k_nearest_n<A, K, MAXK, dev::SqEuclidean>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds,
+                     a.s, dt, q, a.order, a.m, dev::SqEuclidean{}, k, O(radius), excl, items, static_cast<O*>(dists),
+                     counts, t->counters);
   KD_HIP(hipGetLastError());
   return KD_OK;
 }
@@ -229,6 +204,8 @@ int launch_typed(const QueryArgs& a, int k, double radius, uint32_t* items, void
 
 int launch_nearest_n(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists,
                      uint32_t* counts) {
+  if (metric_kind(a.flags) != 0)  // other metrics: the thread-per-query kernel (families.h)
+    KD_WITH_METRIC(a.flags, mlaunch_nearest_n(a, met, k, radius, items, dists, counts));
   KD_DISPATCH_AK(a.t->dtype, a.t->dims, return (launch_typed<A, KK>(a, k, radius, items, dists, counts)));
   return KD_ERR_UNSUPPORTED;
 }

@@ -23,15 +23,17 @@
 namespace kd {
 namespace dev {
 
-template <class O, int K> struct PeriodicQW {
+template <class O, int K, class M = SqEuclidean> struct PeriodicQW {
   Small<O, K> q;    // widened query
   Small<O, K> box;  // widened box lengths
+  M met;
   __device__ O operator[](int d) const { return q[d]; }
 };
 
-// Minimum-image squared distance of entry p, in Kiddo's periodic leaf order.
-template <class A, int K, class O>
-__device__ __forceinline__ O point_dist(const DevTree<A, K>& t, uint32_t p, const PeriodicQW<O, K>& pq) {
+// Minimum-image distance of entry p, in Kiddo's periodic leaf order:
+// acc = combine(acc, dist1(m, 0)) per axis.
+template <class A, int K, class O, class M>
+__device__ __forceinline__ O point_dist(const DevTree<A, K>& t, uint32_t p, const PeriodicQW<O, K, M>& pq) {
   O acc = O(0);
 #pragma unroll
   for (int d = 0; d < K; ++d) {
@@ -39,7 +41,7 @@ __device__ __forceinline__ O point_dist(const DevTree<A, K>& t, uint32_t p, cons
     const O delta = fabs(sub_rn(pq.q[d], c));
     const O wrapped = sub_rn(pq.box[d], delta);
     const O m = wrapped < delta ? wrapped : delta;
-    acc = add_rn(acc, mul_rn(m, m));
+    acc = pq.met.combine(acc, pq.met.d1(m));
   }
   return acc;
 }
@@ -65,9 +67,9 @@ template <class O> __device__ __forceinline__ O periodic_interval_dist(O q, O lo
 // Periodic backtracking. V supplies prune(rd) (the family's rule, canonical: equal
 // distances are only pruned by an exclusive radius) and leaf(tree, pq, leaf_index).
 // Counters: one stem per descent step (sentinel stems included), leaves and entries scanned.
-template <class A, int K, class V>
+template <class A, int K, class V, class M>
 __device__ __forceinline__ void periodic_traverse(const DevTree<A, K>& t, const Small<A, K>& q,
-                                                  const PeriodicQW<typename Ax<A>::O, K>& pq, V& vis,
+                                                  const PeriodicQW<typename Ax<A>::O, K, M>& pq, V& vis,
                                                   Counters& c) {
   using O = typename Ax<A>::O;
   if (t.n == 0) return;
@@ -100,10 +102,8 @@ __device__ __forceinline__ void periodic_traverse(const DevTree<A, K>& t, const 
       const O qd = pq.q[dim], L = pq.box[dim];
       const O n_off = periodic_interval_dist(qd, n_lo, n_hi, L);
       const O f_off = periodic_interval_dist(qd, f_lo, f_hi, L);
-      const O oo = off[dim];
-      const O base = sub_rn(rd, mul_rn(oo, oo));
-      const O n_rd = add_rn(base, mul_rn(n_off, n_off));
-      const O f_rd = add_rn(base, mul_rn(f_off, f_off));
+      const O n_rd = pq.met.rect(rd, off, dim, n_off);
+      const O f_rd = pq.met.rect(rd, off, dim, f_off);
       if (!vis.prune(f_rd)) pending |= 1u << level;
       idx = (idx << 1) | (right ? 1u : 0u);
       ++level;
@@ -144,8 +144,7 @@ __device__ __forceinline__ void periodic_traverse(const DevTree<A, K>& t, const 
         const O n_lo = side ? (ol < pw ? pw : ol) : ol;
         const O n_hi = side ? ou : (ou < pw ? ou : pw);
         const O n_off = periodic_interval_dist(pq.q[d], n_lo, n_hi, pq.box[d]);
-        const O oo = off[d];
-        rd = add_rn(sub_rn(rd, mul_rn(oo, oo)), mul_rn(n_off, n_off));
+        rd = pq.met.rect(rd, off, d, n_off);
         lo.set(d, n_lo);
         hi.set(d, n_hi);
         off.set(d, n_off);

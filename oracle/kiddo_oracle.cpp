@@ -254,6 +254,55 @@ struct Stats {
   uint64_t leaves = 0, points = 0, stems = 0;
 };
 
+// Distance metrics (src/dist/*/mod.rs): dist1 of a difference or offset, the combine of
+// per-axis terms from 0 in axis order, and the rect distance after one axis offset changes.
+//   0 SquaredEuclidean  d*d, sum, (rd - old^2) + new^2     (squared_euclidean/mod.rs:36-39)
+//   1 Manhattan         |d|, sum, (rd - old) + new         (manhattan/mod.rs:36-49)
+//   2 Chebyshev         |d|, max, rescan of the offsets    (chebyshev/mod.rs:34-61)
+//   3 Minkowski<P>      |d|^P, sum, (rd - old^P) + new^P   (minkowski/mod.rs:55-59; powi
+//                       restated as binary exponentiation, the usual expansion of llvm.powi)
+template <class O> struct Metric {
+  int kind = 0, p = 0;
+  O d1(O x) const {
+    switch (kind) {
+      case 1:
+      case 2:
+        return std::fabs(x);
+      case 3: {
+        O a = std::fabs(x), r = O(1);
+        for (int b = p;;) {
+          if (b & 1) r = r * a;
+          b >>= 1;
+          if (!b) break;
+          a = a * a;
+        }
+        return r;
+      }
+      default:
+        return x * x;
+    }
+  }
+  O combine(O acc, O c) const {
+    if (kind == 2) return c > acc ? c : acc;
+    return acc + c;
+  }
+  template <int K> O rect(O rd, const O* off, int dim, O no) const {
+    switch (kind) {
+      case 1:
+        return (rd - off[dim]) + no;
+      case 2: {
+        O acc = O(0);
+        for (int a = 0; a < K; ++a) acc = combine(acc, a == dim ? no : off[a]);
+        return acc;
+      }
+      case 3:
+        return (rd - d1(off[dim])) + d1(no);
+      default:
+        return (rd - off[dim] * off[dim]) + no * no;
+    }
+  }
+};
+
 template <class A, int K> struct View {
   using O = typename Ax<A>::O;
   const A* stems;
@@ -263,6 +312,7 @@ template <class A, int K> struct View {
   uint64_t n;
   int max_stem_level;
   const O* pbox = nullptr;  // periodic box (widened); null: Cartesian
+  Metric<O> met{};
 
   // src/leaf_view/mod.rs:431-446 tile walk; calls f(O dist, uint32_t item) per entry
   // with dist computed as ((0 + dist1(x)) + dist1(y)) + ... in O, no FMA.
@@ -287,7 +337,7 @@ template <class A, int K> struct View {
           } else {
             diff = O(c) - qw[d];
           }
-          acc += diff * diff;
+          acc = met.combine(acc, met.d1(diff));
         }
         uint32_t it;
         std::memcpy(&it, p + size_t(K) * tl * sizeof(A) + size_t(i) * 4, 4);
@@ -354,7 +404,7 @@ template <class A, int K> struct View {
           O pw = O(pivot);
           O new_off = qw[dim] >= pw ? qw[dim] - pw : pw - qw[dim];  // |a-b| (axis.rs:124)
           O old_off = off[dim];
-          O rd_far = (rd - old_off * old_off) + new_off * new_off;
+          O rd_far = met.template rect<K>(rd, off, dim, new_off);
           bool cand = check_bound ? !(rd_far > ctx.max_dist()) : true;
           if (cand) {
             rs.push_back({old_off, true});
@@ -470,9 +520,8 @@ template <class A, int K> struct View {
         const O f_lo = right ? l_lo : r_lo, f_hi = right ? l_hi : r_hi;
         const O n_off = piv(qw[dim], n_lo, n_hi, pbox[dim]);
         const O f_off = piv(qw[dim], f_lo, f_hi, pbox[dim]);
-        const O old = cur.off[dim];
-        const O n_rd = (cur.rd - old * old) + n_off * n_off;
-        const O f_rd = (cur.rd - old * old) + f_off * f_off;
+        const O n_rd = met.template rect<K>(cur.rd, cur.off, dim, n_off);
+        const O f_rd = met.template rect<K>(cur.rd, cur.off, dim, f_off);
         const bool n_pruned = ctx.prune(n_rd), f_pruned = ctx.prune(f_rd);
         if (!f_pruned) {
           Frame f = cur;
@@ -684,6 +733,7 @@ struct StatsAcc {
 
 struct or_tree {
   Layout L;
+  int metric_kind = 0, metric_p = 0;  // metric of the following queries (or_set_metric)
 };
 
 extern "C" {
@@ -742,6 +792,39 @@ int or_from_layout(int dtype, int K, int B, uint64_t n, int max_stem_level, cons
 
 void or_free(or_tree* t) { delete t; }
 
+// The metric functions themselves, for the reference's metric tests (src/dist/mod.rs):
+// the distance of two K-vectors (f64) and the rect distance after one offset changes.
+int or_metric_dist(uint32_t flags, const double* a, const double* b, int K, double* out) {
+  Metric<double> m;
+  m.kind = int((flags >> 8) & 0xF);
+  m.p = int((flags >> 12) & 0xFF);
+  double acc = 0.0;
+  for (int d = 0; d < K; ++d) acc = m.combine(acc, m.d1(a[d] - b[d]));
+  *out = acc;
+  return 0;
+}
+int or_metric_rect(uint32_t flags, double rd, const double* off, int K, int dim, double no, double* out) {
+  Metric<double> m;
+  m.kind = int((flags >> 8) & 0xF);
+  m.p = int((flags >> 12) & 0xFF);
+  switch (K) {
+    case 2: *out = m.rect<2>(rd, off, dim, no); return 0;
+    case 3: *out = m.rect<3>(rd, off, dim, no); return 0;
+    case 4: *out = m.rect<4>(rd, off, dim, no); return 0;
+    default: return -1;
+  }
+}
+
+// Metric of the following queries on this tree: flags bits 8-11 kind (0 SquaredEuclidean,
+// 1 Manhattan, 2 Chebyshev, 3 Minkowski), bits 12-19 the Minkowski exponent.
+int or_set_metric(or_tree* t, uint32_t flags) {
+  const int kind = int((flags >> 8) & 0xF), p = int((flags >> 12) & 0xFF);
+  if (kind > 3 || (kind == 3 && (p == 1 || p == 2))) return -2;
+  t->metric_kind = kind;
+  t->metric_p = p;
+  return 0;
+}
+
 // update_pivot on its own (construction.rs:1089-1175), for its unit test
 // (construction.rs:1187-1213). Returns the pivot; sort_index is reordered in place.
 int64_t or_update_pivot(int dtype, int K, const void* pts, uint64_t n, uint64_t* sort_index,
@@ -773,8 +856,11 @@ const uint8_t* or_arena(const or_tree* t) { return t->L.arena.data(); }
 namespace {
 template <class A, int K> View<A, K> view_of(const or_tree* t) {
   const Layout& L = t->L;
-  return View<A, K>{reinterpret_cast<const A*>(L.stems.data()), L.extents.data(),
-                    L.extents.size() / 2, L.arena.data(), L.n, L.max_stem_level};
+  View<A, K> v{reinterpret_cast<const A*>(L.stems.data()), L.extents.data(),
+               L.extents.size() / 2, L.arena.data(), L.n, L.max_stem_level};
+  v.met.kind = t->metric_kind;
+  v.met.p = t->metric_p;
+  return v;
 }
 }  // namespace
 
@@ -1016,7 +1102,7 @@ int or_periodic_nearest_one(const or_tree* t, const void* q, uint64_t m, const v
           wq[axis] = wc;
           A off = wc < A(0) ? A(std::fabs(wc - A(0))) : wc > L ? A(std::fabs(wc - L)) : A(0);
           O ow = O(off);
-          O nlb = lb + ow * ow;
+          O nlb = v.met.combine(lb, v.met.d1(ow));
           if (!accepts(nlb)) continue;
           rec(axis + 1, shifted || sh != 0, nlb);
         }

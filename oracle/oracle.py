@@ -67,6 +67,9 @@ def lib():
                                             i32, vp]
         L.or_periodic_within_count.argtypes = [vp, vp, u64, vp, f64, i32, vp, i32, vp]
         L.or_periodic_within_fill.argtypes = [vp, vp, u64, vp, f64, i32, i32, i32, vp, vp, vp, i32]
+        L.or_set_metric.argtypes = [vp, C.c_uint32]
+        L.or_metric_dist.argtypes = [C.c_uint32, vp, vp, i32, vp]
+        L.or_metric_rect.argtypes = [C.c_uint32, f64, vp, i32, i32, f64, vp]
         L.or_brute_nearest_n.argtypes = [vp, vp, u64, u64, f64, i32, vp, vp, vp, i32]
         L.or_brute_within.argtypes = [vp, vp, u64, f64, i32, vp, vp, vp, vp, i32]
         L.or_brute_best_n_within.argtypes = [vp, vp, u64, f64, u64, i32, vp, vp, vp, i32]
@@ -81,6 +84,24 @@ def _p(a):
 def _check(rc, what):
     if rc != 0:
         raise RuntimeError(f"oracle {what} failed with status {rc}")
+
+
+def metric_dist(flags: int, a, b) -> float:
+    """Distance of two f64 vectors under the metric the flags name (bits 8-19)."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros(1)
+    _check(lib().or_metric_dist(flags, _p(a), _p(b), len(a), _p(out)), "metric_dist")
+    return float(out[0])
+
+
+def metric_rect(flags: int, rd: float, off, dim: int, new_off: float) -> float:
+    """Rect distance after replacing off[dim] by new_off (rect_dist_after_update)."""
+    off = np.ascontiguousarray(off, np.float64)
+    out = np.zeros(1)
+    _check(lib().or_metric_rect(flags, float(rd), _p(off), len(off), dim, float(new_off), _p(out)),
+           "metric_rect")
+    return float(out[0])
 
 
 def update_pivot(points: np.ndarray, sort_index: np.ndarray, dim: int, init: int) -> int:
@@ -237,6 +258,11 @@ class Tree:
                                     int(sorted), mode, _p(offsets), _p(items), _p(d), threads),
                "within_fill")
         return (offsets, items, d, st) if stats else (offsets, items, d)
+
+    def set_metric(self, flags: int):
+        """Metric of the following queries (flags bits 8-19, as the GPU engine's)."""
+        _check(lib().or_set_metric(self._h, int(flags)), "set_metric")
+        return self
 
     # -- periodic boundary conditions (float trees; box: K positive lengths) -------------
     def _box(self, box):
