@@ -58,7 +58,8 @@ def main():
     ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--no-within", action="store_true", help="knn16: skip the within(r=0.002) pass")
-    ap.add_argument("--workload", choices=["knn16", "forest", "serving", "mixture", "small", "periodic"],
+    ap.add_argument("--workload", choices=["knn16", "forest", "serving", "mixture", "small", "periodic",
+                                           "metrics"],
                     default="knn16",
                     help="knn16: the headline 16M-point k-NN; forest: 4096-tree batched form; "
                          "serving: continuous batching over resident trees")
@@ -94,6 +95,8 @@ def main():
         return bench_small(args, torch, ka)
     if args.workload == "periodic":
         return bench_periodic(args, torch, ka)
+    if args.workload == "metrics":
+        return bench_metrics(args, torch, ka)
 
     if args.copy_probe:
         # A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it scales the
@@ -660,6 +663,71 @@ def bench_periodic(args, torch, ka):
         "config": {"workload": f"periodic-f64-3d-{n}-{m}q", "points": n, "queries": m, "k": k,
                    "box": [1.0, 1.0, 1.0]},
         "families": res, "cpu_baseline": cpu,
+    }), flush=True)
+
+
+def bench_metrics(args, torch, ka):
+    """Other distance metrics on the 16M-point f64 tree of the periodic workload (no
+    duplicates): nearest_n(16) and within under SquaredEuclidean, Manhattan, Chebyshev and
+    Minkowski<3>, Morton-ordered thread-per-query kernels; within at the median k-th
+    distance of the metric, so a median query holds about k entries. CPU baseline: the oracle's
+    nearest_n(16) under Manhattan on a sample."""
+    n, m, k = args.points, args.queries, args.k
+    pts = np.random.default_rng(args.seed).random((n, 3))
+    qh = np.random.default_rng(args.seed + 1).random((m, 3))
+    tree = ka.ImmutableKdTree.new_from_slice(pts, args.bucket)
+    tree.set_timing(True)
+    q = torch.from_numpy(qh).cuda()
+    flags = {"sqeuclidean": 0, "manhattan": ka.KD_METRIC_MANHATTAN, "chebyshev": ka.KD_METRIC_CHEBYSHEV,
+             "minkowski3": ka.KD_METRIC_MINKOWSKI(3)}
+    res = {}
+    for name, f in flags.items():
+        for _ in range(args.warmup):
+            tree.nearest_n_raw(q, k, math.inf, f | ka.KD_MORTON)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kms = []
+        for _ in range(args.steps):
+            out = tree.nearest_n_raw(q, k, math.inf, f | ka.KD_MORTON)
+            kms.append(tree.last_kernel_ms())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        st = tree.query_stats()
+        kth = float(out.distances[:, -1].float().median().item())
+        res[name] = {"nearest_n_ms": round(dt * 1e3, 2), "kernel_ms": round(float(np.mean(kms)), 2),
+                     "qps": round(m / dt, 1), "leaves_per_query": round(st["leaves"] / m, 2),
+                     "median_kth_distance": kth}
+        t0 = time.perf_counter()
+        w = tree.within_raw(q, kth, f | ka.KD_MORTON)
+        torch.cuda.synchronize()
+        res[name]["within_at_median_kth_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        res[name]["within_results"] = int(w.offsets[-1].item())
+        del w, out
+    log(f"metrics: {res}")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.build()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ot = O.Tree.build(pts, args.bucket).set_metric(ka.KD_METRIC_MANHATTAN)
+        ns = min(args.cpu_sample, m)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ot.nearest_n(qh[:ns], k, mode=O.FAITHFUL, threads=threads)
+            runs.append(time.perf_counter() - t0)
+        cdt = float(np.median(runs))
+        cpu = dict(value=round(ns / cdt, 1), unit="queries/s", cores=threads, kind="port",
+                   sample=f"first {ns} of the {m} queries, Manhattan nearest_n(k={k}), Kiddo's query "
+                          f"restated in C++ (oracle/kiddo_oracle.cpp), {threads} threads, median of 3 runs",
+                   seconds=round(cdt, 2))
+    print(json.dumps({
+        "metric": f"queries/s, {n // 1_000_000}M-pt f64 3D Manhattan nearest_n({k})",
+        "value": res["manhattan"]["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["manhattan"]["nearest_n_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform)",
+        "config": {"workload": f"metrics-f64-3d-{n}-{m}q", "points": n, "queries": m, "k": k},
+        "by_metric": res, "cpu_baseline": cpu,
     }), flush=True)
 
 
