@@ -122,6 +122,22 @@ void kd_layout_free(kd_layout* layout);
  * kernels (synchronous). Replaces holding `&ImmutableKdTree` (kiddo src/lib.rs:270-271).
  * Rejects NaN coordinates, leaves over 1024 entries and inconsistent extents. */
 int kd_tree_create(const kd_tree_desc* desc, uint32_t tree_flags, kd_tree** out);
+
+/* ImmutableKdTree::new_from_slice / new_from_entries built on the GPU from points already
+ * in device memory (kiddo src/kd_tree/construction.rs:413-539, :641-691, :868-968,
+ * :1089-1175): level-synchronous radix sorts, one split per node. Same stems (bit for
+ * bit) and the same entries in every leaf as kd_build_layout; the order inside a leaf
+ * may differ (no query observes it). points: device, row-major n x dims; items: device
+ * u32[n] or NULL (item = source index). Same argument checks as kd_build_layout and
+ * kd_tree_create (NaN / sentinel coordinates, leaves over 1024 entries); n < 2^31.
+ * Synchronous on `stream` when it returns. */
+int kd_tree_build_device(int32_t dtype, int32_t dims, int32_t bucket, uint64_t n,
+                         const void* points, const uint32_t* items, uint32_t tree_flags,
+                         kd_tree** out, void* stream);
+/* Copies a device tree back to the host in Kiddo's layout (stems, extents, arena; the
+ * buffers kd_layout_desc describes), e.g. for a Rust shim to own a KdTree built on the
+ * GPU (KdTreeAccessor, src/traits/kd_tree.rs:16-46). Free with kd_layout_free. */
+int kd_tree_export_layout(const kd_tree* tree, kd_layout** out);
 int kd_tree_destroy(kd_tree* tree);
 int kd_tree_get_info(const kd_tree* tree, kd_tree_info* out);
 

@@ -94,6 +94,8 @@ SIGNATURES = {
     "kd_layout_desc": ([_vp, C.POINTER(TreeDesc)], C.c_int),
     "kd_layout_free": ([_vp], None),
     "kd_tree_create": ([C.POINTER(TreeDesc), _u32, C.POINTER(_vp)], C.c_int),
+    "kd_tree_build_device": ([_i32, _i32, _i32, _u64, _vp, _vp, _u32, C.POINTER(_vp), _vp], C.c_int),
+    "kd_tree_export_layout": ([_vp, C.POINTER(_vp)], C.c_int),
     "kd_tree_destroy": ([_vp], C.c_int),
     "kd_tree_get_info": ([_vp, C.POINTER(TreeInfo)], C.c_int),
     "kd_tree_tune": ([_vp, _i32, _i32, _i32], C.c_int),
@@ -183,7 +185,15 @@ class HostLayout:
     """Kiddo-layout tree built on the host (ImmutableKdTree::new_from_slice /
     new_from_entries, construction.rs:278-303, :403-411)."""
 
-    def __init__(self, points: np.ndarray, bucket: int = 32, items=None, threads: int = 0):
+    def __init__(self, points: np.ndarray = None, bucket: int = 32, items=None, threads: int = 0,
+                 _handle=None):
+        if _handle is not None:  # a layout exported from a device tree
+            self._h = _handle
+            self.desc = TreeDesc()
+            _check(lib().kd_layout_desc(self._h, C.byref(self.desc)), "kd_layout_desc")
+            self.dtype_code = self.desc.dtype
+            self.n, self.K, self.bucket = self.desc.size, self.desc.dims, self.desc.bucket
+            return
         pts = np.ascontiguousarray(points)
         if pts.ndim != 2:
             raise ValueError("points must be an (n, K) array")
@@ -260,10 +270,13 @@ class Within:
 class ImmutableKdTree:
     """A Kiddo ImmutableKdTree<A, K> resident in HBM (kiddo src/lib.rs:270-271)."""
 
-    def __init__(self, desc: TreeDesc, shadow_f32: bool = False, keepalive=None):
-        h = C.c_void_p()
-        flags = KD_TREE_SHADOW_F32 if shadow_f32 else 0
-        _check(lib().kd_tree_create(C.byref(desc), flags, C.byref(h)), "kd_tree_create")
+    def __init__(self, desc: TreeDesc, shadow_f32: bool = False, keepalive=None, _handle=None):
+        if _handle is not None:
+            h = _handle
+        else:
+            h = C.c_void_p()
+            flags = KD_TREE_SHADOW_F32 if shadow_f32 else 0
+            _check(lib().kd_tree_create(C.byref(desc), flags, C.byref(h)), "kd_tree_create")
         self._h = h
         self._keep = keepalive
         info = TreeInfo()
@@ -285,6 +298,36 @@ class ImmutableKdTree:
                          shadow_f32: bool = False, threads: int = 0):
         lay = HostLayout(points, bucket, items, threads)
         return cls(lay.desc, shadow_f32, keepalive=None)
+
+    @classmethod
+    def build_on_device(cls, points, bucket: int = 32, items=None, shadow_f32: bool = False,
+                        stream=None):
+        """new_from_slice / new_from_entries built on the GPU (kd_tree_build_device) from a
+        cuda tensor (or numpy array, copied to the device) of shape (n, K)."""
+        torch = _torch()
+        if isinstance(points, np.ndarray):
+            points = torch.from_numpy(np.ascontiguousarray(points)).to("cuda")
+        points = points.contiguous()
+        code = {torch.float32: KD_F32, torch.float64: KD_F64, torch.uint16: KD_U16,
+                torch.uint32: KD_U32}[points.dtype]
+        n, K = points.shape
+        it = None
+        if items is not None:
+            if isinstance(items, np.ndarray):
+                items = torch.from_numpy(np.ascontiguousarray(items, dtype=np.uint32)).to("cuda")
+            it = items.contiguous()
+        h = C.c_void_p()
+        _check(lib().kd_tree_build_device(code, K, bucket, n, points.data_ptr() if n else None,
+                                          it.data_ptr() if it is not None and n else None,
+                                          KD_TREE_SHADOW_F32 if shadow_f32 else 0, C.byref(h),
+                                          _stream_ptr(stream)), "kd_tree_build_device")
+        return cls(None, _handle=h)
+
+    def export_layout(self) -> "HostLayout":
+        """Kiddo's host layout (stems, extents, arena) of this device tree."""
+        h = C.c_void_p()
+        _check(lib().kd_tree_export_layout(self._h, C.byref(h)), "kd_tree_export_layout")
+        return HostLayout(_handle=h)
 
     @classmethod
     def from_layout(cls, dtype, K, bucket, size, max_stem_level, stems, extents, arena,

@@ -261,6 +261,65 @@ template <class A, int K> int create_typed(const kd_tree_desc* desc, uint32_t fl
   return KD_OK;
 }
 
+// Kiddo's host layout of a device tree (stems, per-leaf extents, arena of tiles), the
+// format kd_tree_create takes and a Rust shim hands to KdTree (vec_of_arenas.rs:292-323).
+template <class A, int K> int export_typed(const kd_tree* t, HostLayout& L) {
+  const uint64_t n = t->size, leaves = t->leaf_count;
+  std::vector<uint32_t> ls(leaves + 1, 0);
+  std::vector<A> coords(t->plane * K);
+  std::vector<uint32_t> items(n);
+  auto down = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes == 0) return KD_OK;
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(KD_ERR_DEVICE, "download failed: %s", hipGetErrorString(e));
+    return KD_OK;
+  };
+  L.dtype = t->dtype;
+  L.dims = K;
+  L.bucket = t->bucket;
+  L.size = n;
+  L.max_stem_level = t->max_stem_level;
+  L.stems_len = t->stems_len;
+  L.stems.assign(t->stems_len * sizeof(A), 0);
+  if (int rc = down(L.stems.data(), t->stems, L.stems.size())) return rc;
+  if (int rc = down(ls.data(), t->leaf_start, ls.size() * 4)) return rc;
+  if (int rc = down(coords.data(), t->coords, coords.size() * sizeof(A))) return rc;
+  if (int rc = down(items.data(), t->items, items.size() * 4)) return rc;
+  L.extents.assign(2 * leaves, 0);
+  uint64_t off = 0, max_len = 0;
+  for (uint64_t l = 0; l < leaves; ++l) {
+    off = (off + 7) & ~uint64_t(7);
+    const uint64_t len = ls[l + 1] - ls[l];
+    L.extents[2 * l] = off;
+    L.extents[2 * l + 1] = len;
+    off += len * (K * sizeof(A) + 4);
+    max_len = std::max(max_len, len);
+  }
+  L.max_leaf_len = max_len;
+  L.arena.assign(off, 0);
+  host_parallel(leaves, [&](uint64_t s, uint64_t e) {
+    for (uint64_t l = s; l < e; ++l) {
+      uint8_t* dst = L.arena.data() + L.extents[2 * l];
+      const uint64_t b = ls[l], len = ls[l + 1] - b;
+      for (uint64_t base = 0; base < len;) {
+        const uint64_t rem = len - base;
+        const int tl = rem >= 32 ? 32 : rem >= 8 ? 8 : rem >= 4 ? 4 : rem >= 2 ? 2 : 1;
+        for (int d = 0; d < K; ++d)
+          for (int i = 0; i < tl; ++i) {
+            std::memcpy(dst, &coords[d * t->plane + b + base + i], sizeof(A));
+            dst += sizeof(A);
+          }
+        for (int i = 0; i < tl; ++i) {
+          std::memcpy(dst, &items[b + base + i], 4);
+          dst += 4;
+        }
+        base += tl;
+      }
+    }
+  });
+  return KD_OK;
+}
+
 // Optional Morton ordering of a batch: returns a device permutation (or null).
 struct Order {
   uint32_t* perm = nullptr;
@@ -369,6 +428,45 @@ int kd_tree_create(const kd_tree_desc* desc, uint32_t tree_flags, kd_tree** out)
     return kd::set_error(KD_ERR_NO_MEMORY, "host allocation failed while uploading");
   }
   return KD_ERR_INVALID;
+}
+
+int kd_tree_build_device(int32_t dtype, int32_t dims, int32_t bucket, uint64_t n, const void* points,
+                         const uint32_t* items, uint32_t tree_flags, kd_tree** out, void* stream) {
+  if (!out) return kd::set_error(KD_ERR_INVALID, "out is null");
+  kd_tree* t = nullptr;
+  try {
+    t = new kd_tree;
+  } catch (const std::bad_alloc&) {
+    return kd::set_error(KD_ERR_NO_MEMORY, "host allocation failed");
+  }
+  const int rc = kd::build_device(t, dtype, dims, bucket, n, points, items, tree_flags,
+                                  static_cast<hipStream_t>(stream));
+  if (rc != KD_OK) {
+    kd::free_tree(t);
+    return rc;
+  }
+  *out = t;
+  return KD_OK;
+}
+
+int kd_tree_export_layout(const kd_tree* t, kd_layout** out) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (!out) return kd::set_error(KD_ERR_INVALID, "out is null");
+  kd_layout* l = nullptr;
+  try {
+    l = new kd_layout;
+    int rc = KD_OK;
+    KD_DISPATCH_AK(t->dtype, t->dims, rc = kd::export_typed<A, KK>(t, l->L));
+    if (rc != KD_OK) {
+      delete l;
+      return rc;
+    }
+  } catch (const std::bad_alloc&) {
+    delete l;
+    return kd::set_error(KD_ERR_NO_MEMORY, "host allocation failed while exporting");
+  }
+  *out = l;
+  return KD_OK;
 }
 
 int kd_tree_destroy(kd_tree* tree) {

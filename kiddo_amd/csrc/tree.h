@@ -93,6 +93,10 @@ struct QueryArgs {
 
 constexpr int kMaxK = 64;
 
+// k_build.hip: fills t (dtype, dims, bucket, size set here) from device points
+int build_device(kd_tree* t, int dtype, int dims, int bucket, uint64_t n, const void* points,
+                 const uint32_t* items, uint32_t flags, hipStream_t s);
+
 // k_knn.hip
 int launch_nearest_n(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists,
                      uint32_t* counts);
