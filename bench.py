@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--no-within", action="store_true", help="knn16: skip the within(r=0.002) pass")
+    ap.add_argument("--no-device-build", action="store_true", help="knn16: skip timing the GPU tree build")
     ap.add_argument("--workload", choices=["knn16", "forest", "serving", "mixture", "small", "periodic",
                                            "metrics"],
                     default="knn16",
@@ -115,7 +116,8 @@ def main():
     log(f"data {pts.shape[0]} points, {qh.shape[0]} queries in {time.time() - t0:.1f}s")
     t0 = time.time()
     lay = ka.HostLayout(pts, args.bucket)
-    log(f"host build {time.time() - t0:.1f}s, stems depth {lay.max_stem_level + 1}")
+    host_build_s = time.time() - t0
+    log(f"host build {host_build_s:.1f}s, stems depth {lay.max_stem_level + 1}")
     t0 = time.time()
     tree = ka.ImmutableKdTree(lay.desc, shadow_f32=True)
     tree.set_timing(True)
@@ -220,6 +222,24 @@ def main():
                     kernel_ms=round(results[head]["kernel_ms"], 3),
                     algo_bytes_per_launch=int(algo_bytes))
 
+    # construction on the GPU from the points already in HBM (kd_tree_build_device): the
+    # second of two builds is timed; its stems must equal the host builder's bit for bit
+    build = None
+    if not args.no_device_build:
+        dpts = torch.from_numpy(pts).to(dev)
+        torch.cuda.synchronize()
+        ka.ImmutableKdTree.build_on_device(dpts, args.bucket)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gt = ka.ImmutableKdTree.build_on_device(dpts, args.bucket)
+        torch.cuda.synchronize()
+        gdt = time.perf_counter() - t0
+        same = bool(gt.export_layout().stems().tobytes() == lay.stems().tobytes())
+        build = dict(device_ms=round(gdt * 1e3, 1), host_ms=round(host_build_s * 1e3, 1),
+                     host_threads=os.cpu_count(), stems_equal_host=same)
+        log(f"device build {gdt * 1e3:.1f} ms (host {host_build_s * 1e3:.0f} ms), stems equal: {same}")
+        del gt, dpts
+
     # compulsory footprint: tree bytes + queries + results, once each, over the kernel time
     footprint = (tree.info.device_bytes + m * 3 * 8 + m * k * (4 + 8) + m * 4)
     compulsory = dict(bytes=int(footprint), gbs=round(footprint / kern_s / 1e9, 1))
@@ -296,6 +316,7 @@ def main():
         "leaves_per_query": round(st["leaves"] / m, 2),
         "entries_per_query": round(st["points"] / m, 2),
         "compulsory_footprint": compulsory,
+        "tree_build": build,
         "within_r0.002_sorted": within,
         "roofline": roofline,
         "cpu_baseline": cpu,
