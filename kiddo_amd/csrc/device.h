@@ -516,12 +516,19 @@ template <class O, int MAXK> struct BestNVis {
 //   The FP32 sum of squares of non-negative terms is at most ||delta||^2 (1+u)^K, and the
 //   exact distance is at most the FP64 one times (1 + 1e-15). So FP64 dist <= T implies
 //   FP32 dist <= (sqrt(T (1+1e-15)) + sqrt(K) eps)^2 (1 + 4u) + 1e-44, rounded up.
+// The square is expanded as T' + 2 e s + e^2 with s >= sqrt(T'): an FP32 square root of T'
+// rounded up to FP32, inflated by 2^-19 (far above any FP32 square root's error), so the
+// threshold costs one FP32 square root instead of an FP64 one.
+__device__ __forceinline__ float next_up(float f) { return __uint_as_float(__float_as_uint(f) + 1u); }
 __device__ __forceinline__ float screen_threshold(double T, double eps_k) {
   if (!(T < __builtin_huge_val())) return __builtin_huge_valf();
-  const double r = sqrt(T * (1.0 + 1e-15)) + eps_k;
-  const double tt = r * r * (1.0 + 4.0 * 0x1p-24) + 1e-44;
+  const double tp = T * (1.0 + 1e-15);
+  float tf = (float)tp;
+  if ((double)tf < tp) tf = next_up(tf);  // tf >= tp (tp >= 0)
+  const double s = double(sqrtf(tf)) * (1.0 + 0x1p-19);
+  const double tt = (tp + 2.0 * eps_k * s + eps_k * eps_k) * (1.0 + 4.0 * 0x1p-24) + 1e-44;
   float f = (float)tt;
-  if ((double)f < tt) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f > 0)
+  if ((double)f < tt) f = next_up(f);
   return f;
 }
 
@@ -607,10 +614,20 @@ constexpr int kBlock = 256;
 
 // Kernel prologue shared by every family: one thread per query, queries optionally
 // visited in a permuted (Morton) order; results always land at the query's own index.
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i to XCD i mod 8), and
+// each XCD has its own L2. When queries run in Morton order, neighbouring blocks share
+// stems and leaves, so block i takes the logical block that gives XCD (i mod 8) one
+// contiguous run of the batch (a bijection on [0, gridDim.x)).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t g) {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t q = g / kXcds, r = g % kXcds, x = i % kXcds;
+  return x * q + (x < r ? x : r) + i / kXcds;
+}
+
 #define KD_QUERY_PROLOGUE                                             \
   using O = typename Ax<A>::O;                                        \
   pin_stems(t);                                                       \
-  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;     \
+  const uint64_t i = uint64_t(order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x; \
   const bool active = i < m;                                          \
   const uint64_t qi = active ? (order ? order[i] : i) : 0;            \
   Small<A, K> q;                                                      \
