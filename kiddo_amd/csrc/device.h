@@ -346,10 +346,12 @@ template <class O, int MAXK> struct TopK {
     return ad < bd || (ad == bd && ai < bi);
   }
   // inserts c, dropping the last slot
-  __device__ __forceinline__ void insert(O cd, uint32_t ci) {
+  __device__ __forceinline__ void insert(O cd, uint32_t ci) { insert_if(true, cd, ci); }
+  // inserts c when ok (a predicate rather than a branch: no copy of the slots is made)
+  __device__ __forceinline__ void insert_if(bool ok, O cd, uint32_t ci) {
     bool lt[MAXK];
 #pragma unroll
-    for (int j = 0; j < MAXK; ++j) lt[j] = less(cd, ci, d[j], it[j]);
+    for (int j = 0; j < MAXK; ++j) lt[j] = ok && less(cd, ci, d[j], it[j]);
 #pragma unroll
     for (int j = MAXK - 1; j > 0; --j) {
       const bool sh = lt[j - 1];
@@ -382,6 +384,9 @@ template <class O, int MAXK> struct TopK {
 // nearest_n / nearest_n().within (nearest_n_within.rs:441-479): bound = min(k-th distance
 // when full, radius); leaf filter `<= radius` (`<` exclusive); a full collector accepts a
 // candidate that is smaller in (distance, item) order.
+// `cap` (default +inf) is an optional certified upper bound on the k-th distance, known
+// before the traversal (k entries at most that far exist): entries and subtrees beyond it
+// cannot reach the result, so it bounds the search from the start. Equal distances stay in.
 template <class O, int MAXK> struct KnnVis {
   TopK<O, MAXK> top;
   int k, cnt;
@@ -389,6 +394,7 @@ template <class O, int MAXK> struct KnnVis {
   bool excl;
   O kd;
   uint32_t kit;
+  O cap;
   __device__ void init(int k_, O r, bool ex) {
     top.init();
     k = k_;
@@ -397,26 +403,45 @@ template <class O, int MAXK> struct KnnVis {
     excl = ex;
     kd = oinf<O>();
     kit = 0xFFFFFFFFu;
+    cap = oinf<O>();
   }
-  __device__ O bound() const { return (cnt >= k && kd < radius) ? kd : radius; }
+  __device__ O bound() const {
+    const O b = (cnt >= k && kd < radius) ? kd : radius;
+    return cap < b ? cap : b;
+  }
   __device__ bool prune(O rd) const { return rd > bound() || (excl && !(rd < radius)); }
-  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  __device__ bool unbounded() const { return radius == oinf<O>() && cap == oinf<O>(); }
   __device__ __forceinline__ void offer(O d, uint32_t it) {
     top.insert(d, it);
     cnt = cnt < k ? cnt + 1 : k;
     if (cnt >= k) top.kth(k, kd, kit);
   }
+  __device__ __forceinline__ void offer_if(bool ok, O d, uint32_t it) {
+    top.insert_if(ok, d, it);
+    cnt = ok ? (cnt < k ? cnt + 1 : k) : cnt;
+    if (cnt >= k) top.kth(k, kd, kit);
+  }
+  // The distances of kChunk entries are computed in one branch-free block (indices clamped
+  // to the leaf), so their coordinate loads are in flight together; candidates then load
+  // their item and are offered in entry order.
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
+    constexpr uint32_t kChunk = 4;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
-    for (uint32_t p = s; p < e; ++p) {
-      const O d = point_dist(t, p, qw);
-      const bool ok = excl ? d < radius : d <= radius;
-      if (!ok) continue;
-      if (cnt >= k && !(d <= kd)) continue;
-      const uint32_t it = t.items[p];
-      if (cnt >= k && !TopK<O, MAXK>::less(d, it, kd, kit)) continue;
-      offer(d, it);
+    for (uint32_t p0 = s; p0 < e; p0 += kChunk) {
+      O dd[kChunk];
+#pragma unroll
+      for (uint32_t u = 0; u < kChunk; ++u) dd[u] = point_dist(t, p0 + u < e ? p0 + u : e - 1, qw);
+#pragma unroll
+      for (uint32_t u = 0; u < kChunk; ++u) {
+        const uint32_t p = p0 + u;
+        const O d = dd[u];
+        const bool cand = p < e && (excl ? d < radius : d <= radius) && d <= cap && (cnt < k || d <= kd);
+        if (cand) {
+          const uint32_t it = t.items[p];
+          offer_if(cnt < k || TopK<O, MAXK>::less(d, it, kd, kit), d, it);
+        }
+      }
     }
   }
 };
@@ -545,37 +570,46 @@ template <int MAXK> struct ScreenKnnVis {
   // against the threshold at the start of the run, then only survivors are measured in
   // FP64 and offered (in entry order) to the collector. The wave then pays the FP64 and
   // insertion work once per survivor of its busiest lane, not once per scanned entry.
+  // The screen issues the shadow loads of 8 entries before using any (indices clamped to
+  // the leaf, so every load is in bounds), a survivor's coordinates and item are loaded
+  // together, and the insertion is predicated rather than branched.
   template <int K, class QW>
   __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf) {
+    constexpr uint32_t kBatch = 8;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     for (uint32_t c0 = s; c0 < e; c0 += 32) {
       const uint32_t n = (e - c0) < 32u ? (e - c0) : 32u;
-      uint32_t mask = 0;
-      for (uint32_t j = 0; j < n; ++j) {
-        const float4 sh = t.shadow[c0 + j];
-        const float dx = sub_rn(sh.x, qf[0]);
-        float acc = mul_rn(dx, dx);
-        const float dy = sub_rn(sh.y, qf[1]);
-        acc = add_rn(acc, mul_rn(dy, dy));
-        if (K > 2) {
-          const float dz = sub_rn(sh.z, qf[2]);
-          acc = add_rn(acc, mul_rn(dz, dz));
+      // unbounded (top-k not yet full, no radius): every entry survives the screen
+      const bool screen = thr < __builtin_huge_valf();
+      uint32_t mask = screen ? 0u : (n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u);
+      for (uint32_t j0 = 0; screen && j0 < n; j0 += kBatch) {
+        float4 sh[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) sh[u] = t.shadow[c0 + (j0 + u < n ? j0 + u : n - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+          const float dx = sub_rn(sh[u].x, qf[0]);
+          float acc = mul_rn(dx, dx);
+          const float dy = sub_rn(sh[u].y, qf[1]);
+          acc = add_rn(acc, mul_rn(dy, dy));
+          if (K > 2) {
+            const float dz = sub_rn(sh[u].z, qf[2]);
+            acc = add_rn(acc, mul_rn(dz, dz));
+          }
+          mask |= ((j0 + u < n) && !(acc > thr) ? 1u : 0u) << (j0 + u);
         }
-        mask |= (acc > thr ? 0u : 1u) << j;
       }
       while (mask) {
         const uint32_t j = uint32_t(__ffs(mask) - 1);
         mask &= mask - 1u;
         const uint32_t p = c0 + j;
         ++rescored;
-        const double d = point_dist(t, p, qw);
-        const bool ok = v.excl ? d < v.radius : d <= v.radius;
-        if (!ok) continue;
-        if (v.cnt >= v.k && !(d <= v.kd)) continue;
         const uint32_t it = t.items[p];
-        if (v.cnt >= v.k && !TopK<double, MAXK>::less(d, it, v.kd, v.kit)) continue;
-        v.offer(d, it);
-        thr = screen_threshold(v.bound(), eps_k);
+        const double d = point_dist(t, p, qw);
+        bool ok = (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
+        ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
+        v.offer_if(ok, d, it);
+        thr = ok ? screen_threshold(v.bound(), eps_k) : thr;
       }
     }
   }

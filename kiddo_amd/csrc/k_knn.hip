@@ -30,6 +30,73 @@ __device__ __forceinline__ void screen_setup(const Small<double, K>& q, const do
   eps_k = sqrt((double)K) * eps * (1.0 + 1e-6);
 }
 
+// Certified upper bound on the k-th smallest FP64 distance from q, found before the
+// traversal: the FP32 distances of the entries of the query's own block of 2^j leaves
+// (the smallest block holding 4k entries, leaves under one stem are contiguous) give a
+// k-th smallest S32, and every one of those k entries has an FP64 distance of at most
+// (sqrt(S32 (1+4u)) + sqrt(K) eps)^2 (1 + 2e-15): the FP32 sum of non-negative squares is
+// at least ||delta||^2 (1-u)^K, each FP32 coordinate difference is within eps of the
+// exact one, and the FP64 evaluation is within (1 + 1e-15) of the exact square. Returns
+// +inf when the tree holds fewer than k entries.
+template <int K, int MAXK>
+__device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K>& q, const float (&qf)[3],
+                              double eps_k, int k) {
+  if (t.n < uint32_t(k)) return __builtin_huge_val();
+  const int D = t.nlev;
+  uint32_t idx = 1;
+  int dim = 0;
+  for (int level = 0; level < D; ++level) {
+    const double pivot = stem_at(t, idx);
+    idx = (idx << 1) | (q[dim] >= pivot ? 1u : 0u);
+    dim = (dim + 1 == K) ? 0 : dim + 1;
+  }
+  const uint32_t leaf = idx - (1u << D);
+  uint32_t b = 0, e = 0;
+  for (int up = 0;; ++up) {
+    const uint32_t first = (leaf >> up) << up;
+    b = t.leaf_start[first];
+    e = t.leaf_start[first + (1u << up)];
+    if (e - b >= uint32_t(4 * k) || up >= D) break;
+  }
+  if (e - b < uint32_t(k)) return __builtin_huge_val();
+  float top[MAXK];
+#pragma unroll
+  for (int j = 0; j < MAXK; ++j) top[j] = __builtin_huge_valf();
+  constexpr uint32_t kBatch = 8;
+  for (uint32_t p0 = b; p0 < e; p0 += kBatch) {
+    float d32[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) {
+      const float4 sh = t.shadow[p0 + u < e ? p0 + u : e - 1];
+      const float dx = sub_rn(sh.x, qf[0]);
+      float acc = mul_rn(dx, dx);
+      const float dy = sub_rn(sh.y, qf[1]);
+      acc = add_rn(acc, mul_rn(dy, dy));
+      if (K > 2) {
+        const float dz = sub_rn(sh.z, qf[2]);
+        acc = add_rn(acc, mul_rn(dz, dz));
+      }
+      d32[u] = p0 + u < e ? acc : __builtin_huge_valf();
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) {
+      const float c = d32[u];
+      bool lt[MAXK];
+#pragma unroll
+      for (int j = 0; j < MAXK; ++j) lt[j] = c < top[j];
+#pragma unroll
+      for (int j = MAXK - 1; j > 0; --j) top[j] = lt[j - 1] ? top[j - 1] : (lt[j] ? c : top[j]);
+      top[0] = lt[0] ? c : top[0];
+    }
+  }
+  float kth = top[0];
+#pragma unroll
+  for (int j = 1; j < MAXK; ++j) kth = (j == k - 1) ? top[j] : kth;
+  if (!(kth < __builtin_huge_valf())) return __builtin_huge_val();
+  const double r = sqrt(double(kth) * (1.0 + 4.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
+  return r * r * (1.0 + 2e-15);
+}
+
 template <int K, int MAXK, int WAVES>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_nearest_n_screen(
     DevTree<double, K> t, const double* __restrict__ queries, const uint32_t* __restrict__ order,
@@ -42,6 +109,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     ScreenKnnVis<MAXK> v;
     v.v.init(k, radius, excl != 0);
     screen_setup<K>(q, max_abs, v.qf, v.eps_k);
+    v.v.cap = prescan_cap<K, MAXK>(t, q, v.qf, v.eps_k, k);
     v.thr = screen_threshold(v.v.bound(), v.eps_k);
     v.rescored = 0;
     traverse(t, q, qw, v, cnt);

@@ -228,8 +228,10 @@ def test_builder_surface(ka, cuda):
 
 
 def test_traversal_visits_match_oracle(ka, oracle_mod, cuda):
-    """The kernels visit exactly the leaves (and scan exactly the entries) that the
-    oracle's restatement of Kiddo's DFS visits for the same queries."""
+    """The plain and group kernels visit exactly the leaves (and scan exactly the entries)
+    that the oracle's restatement of Kiddo's DFS visits for the same queries. The
+    thread-per-query FP32-screened kernel starts its DFS from a certified bound on the k-th
+    distance, so it visits at most those leaves (its results are checked elsewhere)."""
     _announce("visit counters")
     rng = np.random.default_rng(12)
     pts = rng.random((100_000, 3))
@@ -242,6 +244,10 @@ def test_traversal_visits_match_oracle(ka, oracle_mod, cuda):
         for flags in (0, ka.KD_MORTON, ka.KD_SCREEN_F32 | ka.KD_MORTON):
             t.nearest_n_raw(q, 16, flags=flags)
             st = t.query_stats()
+            if variant == 1 and flags & ka.KD_SCREEN_F32:
+                assert 0 < st["leaves"] <= int(ost[0]), (variant, flags, st, ost)
+                assert 0 < st["points"] <= int(ost[1]), (variant, flags, st, ost)
+                continue
             assert st["leaves"] == int(ost[0]), (variant, flags, st, ost)
             assert st["points"] == int(ost[1]), (variant, flags, st, ost)
     t.tune(0, 0, -1)
