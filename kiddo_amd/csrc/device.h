@@ -68,12 +68,12 @@ template <class O> __device__ __forceinline__ O abs_diff(O a, O b) {
 // compare-and-select; named scalars or an array let the optimiser fold the selects back
 // into an address computation on a stack slot, which spills the vector to scratch memory.
 template <class T, int K> struct Small {
-  static_assert(K >= 1 && K <= 4, "K must be 1..4");
-  typedef T vec4 __attribute__((ext_vector_type(4)));
-  vec4 v;
+  static_assert(K >= 2 && K <= 4, "K must be 2..4");
+  typedef T vec __attribute__((ext_vector_type(K)));
+  vec v;
   __device__ __forceinline__ T operator[](int d) const { return v[d]; }
   __device__ __forceinline__ void set(int d, T x) { v[d] = x; }
-  __device__ __forceinline__ void fill(T x) { v = vec4(x); }
+  __device__ __forceinline__ void fill(T x) { v = vec(x); }
 };
 
 constexpr int kMaxLevels = 32;
@@ -575,7 +575,7 @@ template <int MAXK> struct ScreenKnnVis {
   // together, and the insertion is predicated rather than branched.
   template <int K, class QW>
   __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf) {
-    constexpr uint32_t kBatch = 8;
+    constexpr uint32_t kBatch = 4;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
     for (uint32_t c0 = s; c0 < e; c0 += 32) {
       const uint32_t n = (e - c0) < 32u ? (e - c0) : 32u;
