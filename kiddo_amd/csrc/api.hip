@@ -39,6 +39,19 @@ namespace dev {
 
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 
+// Autotuning sample: `runs` contiguous runs of the batch's Morton order, evenly spaced, so
+// the sample keeps the spatial coherence of the Morton-ordered launch it stands for.
+template <class A, int K>
+__global__ void k_gather_sample(const A* __restrict__ queries, const uint32_t* __restrict__ perm, uint64_t m,
+                                uint64_t ms, uint64_t runs, A* __restrict__ out) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= ms) return;
+  const uint64_t run_len = ms / runs, c = i / run_len, j = i % run_len;
+  const uint64_t src = perm[c * (m / runs) + j];
+#pragma unroll
+  for (int d = 0; d < K; ++d) out[i * K + d] = queries[src * K + d];
+}
+
 // Clears n counters (a kernel rather than hipMemsetAsync so that callers may capture it).
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
@@ -733,9 +746,40 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
   if (flags & KD_SCREEN_F32)
     for (int w = 4; w <= 6; ++w) cands.push_back({1, 0, -1, w});
   if (g0 < 64) cands.push_back({2, 2 * g0, -1, 0});
+  // A Morton-ordered batch is sampled along its own Morton order (16 runs), not by its
+  // first queries in input order, which are spatially sparse and tune for the wrong shape.
+  void* sample_mem = nullptr;
+  const void* sample = queries;
+  constexpr uint64_t kRuns = 16;
+  if ((flags & KD_MORTON) && m > ms && ms % kRuns == 0) {
+    int src = KD_OK;
+    KD_DISPATCH_AK(t->dtype, t->dims, {
+      kd::Order order;
+      src = kd::make_order<A, KK>(t, static_cast<const A*>(queries), m, flags, s, order);
+      if (src == KD_OK && order.perm) {
+        if (hipMalloc(&sample_mem, ms * KK * sizeof(A)) != hipSuccess) {
+          sample_mem = nullptr;
+          src = kd::set_error(KD_ERR_NO_MEMORY, "hipMalloc failed");
+        } else {
+          hipLaunchKernelGGL((kd::dev::k_gather_sample<A, KK>), dim3(kd::grid_for(ms)), dim3(kd::dev::kBlock), 0, s,
+                             static_cast<const A*>(queries), order.perm, m, ms, kRuns, static_cast<A*>(sample_mem));
+          if (hipGetLastError() != hipSuccess) src = kd::set_error(KD_ERR_DEVICE, "sample gather failed");
+          (void)hipStreamSynchronize(s);
+          sample = sample_mem;
+        }
+      }
+    });
+    if (src != KD_OK) {
+      (void)hipFree(sample_mem);
+      return src;
+    }
+  }
   void* mem = nullptr;
   const size_t osz = ms * k * (sizeof(uint32_t) + sizeof(double)) + ms * sizeof(uint32_t) + 1024;
-  KD_HIP(hipMalloc(&mem, osz));
+  if (hipMalloc(&mem, osz) != hipSuccess) {
+    (void)hipFree(sample_mem);
+    return kd::set_error(KD_ERR_NO_MEMORY, "hipMalloc(%zu) failed", osz);
+  }
   uint32_t* items = static_cast<uint32_t*>(mem);
   double* dists = reinterpret_cast<double*>(static_cast<uint8_t*>(mem) + ((ms * k * 4 + 255) / 256) * 256);
   uint32_t* counts = reinterpret_cast<uint32_t*>(dists + ms * k);
@@ -753,7 +797,7 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
     t->waves = c.waves;
     float ms_c = 0.0f;
     for (int rep = 0; rep < 2 && rc == KD_OK; ++rep) {
-      rc = kd_nearest_n(t, queries, ms, k, radius, flags, items, dists, counts, stream);
+      rc = kd_nearest_n(t, sample, ms, k, radius, flags, items, dists, counts, stream);
       if (rc == KD_OK) rc = kd_last_kernel_ms(t, &ms_c);
     }
     if (rc == KD_OK && ms_c < best_ms) {
@@ -763,6 +807,7 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
   }
   (void)hipStreamSynchronize(s);
   (void)hipFree(mem);
+  (void)hipFree(sample_mem);
   t->timing = save_timing;
   if (rc) {
     t->knn_variant = save_variant;
