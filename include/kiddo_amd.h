@@ -152,8 +152,9 @@ int kd_tree_tune2(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_
                   int32_t waves);
 
 /* Autotuner: times each candidate launch shape (thread vs group per query, group size,
- * LDS stem levels, occupancy floor) on up to 2^18 of the given queries with the given
- * k/radius/flags, keeps the fastest in the tree (as kd_tree_tune2 would) and reports it in
+ * LDS stem levels, occupancy floor) on up to 2^20 of the given queries (with KD_MORTON:
+ * 16 contiguous runs of their Morton order) with the given k/radius/flags, keeps the
+ * fastest in the tree (as kd_tree_tune2 would) and reports it in
  * chosen4 = {variant, group_size, lds_levels, waves}. Synchronous. */
 int kd_autotune_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k,
                           double radius, uint32_t flags, int32_t* chosen4, void* stream);

@@ -629,7 +629,14 @@ int kd_within_fill(kd_tree* t, const void* queries, uint64_t m, double radius, u
   if (!queries || !offsets) return kd::set_error(KD_ERR_INVALID, "null buffer");
   if (int rc = kd::check_radius(radius)) return rc;
   KD_ORDERED(a, queries, m, flags, stream);
-  return kd::launch_within_fill(a, radius, offsets, items, dists);
+  // dense outputs (>= 32 results per query on average) take the group-per-query fill,
+  // whose lanes write consecutive results
+  uint64_t total = 0;
+  KD_HIP(hipMemcpyAsync(&total, offsets + m, sizeof(uint64_t), hipMemcpyDeviceToHost, s_));
+  KD_HIP(hipStreamSynchronize(s_));
+  kd::QueryArgs ad = a;
+  ad.dense = total >= 32 * m;
+  return kd::launch_within_fill(ad, radius, offsets, items, dists);
 }
 
 int kd_best_n_within(kd_tree* t, const void* queries, uint64_t m, double radius, uint32_t n, uint32_t flags,
@@ -736,7 +743,9 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
   if (k == 0 || k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_INVALID, "k out of range");
   if (!queries) return kd::set_error(KD_ERR_INVALID, "queries is null");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const uint64_t ms = std::min<uint64_t>(m, uint64_t(1) << 18);
+  // 2^20 queries: a thread-per-query candidate then still fills the device (256 CUs x 2048
+  // lanes); a smaller sample under-occupies it and favours the group forms
+  const uint64_t ms = std::min<uint64_t>(m, uint64_t(1) << 20);
   if (ms == 0) return KD_OK;
   // candidate launch shapes: thread per query, and groups of the two smallest sizes
   // holding k, each with and without LDS-pinned stems

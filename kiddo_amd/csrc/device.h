@@ -775,6 +775,46 @@ __device__ __forceinline__ void group_traverse(const DevTree<A, K>& t, const Sma
   }
 }
 
+// Group within collector (fill pass for dense outputs): the G lanes of a query scan a
+// leaf G entries at a time, and the entries in range are written in entry order, lane j
+// at the query's next free slot plus the number of hits in lanes below j, so consecutive
+// lanes store consecutive results (coalesced) instead of one lane writing them one by one.
+template <class O, int G> struct GroupWithinVis {
+  O radius;
+  bool excl;
+  int gl;
+  uint64_t count;  // results so far (group-uniform)
+  uint32_t* out_items;
+  O* out_d;
+  uint64_t cap;
+  __device__ O bound() const { return radius; }
+  __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    const int base = int(threadIdx.x & 63u) - gl;  // first lane of this group in the wave
+    for (uint32_t p0 = s; p0 < e; p0 += G) {
+      const uint32_t p = p0 + uint32_t(gl);
+      O d = O(0);
+      bool hit = false;
+      if (p < e) {
+        d = point_dist(t, p, qw);
+        hit = excl ? d < radius : d <= radius;
+      }
+      const uint64_t mine = (uint64_t(__ballot(hit)) >> base) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull));
+      if (hit) {
+        const uint64_t o = count + uint64_t(__popcll(mine & ((1ull << gl) - 1ull)));
+        if (o < cap) {
+          out_items[o] = t.items[p];
+          out_d[o] = d;
+        }
+      }
+      count += uint64_t(__popcll(mine));
+    }
+  }
+};
+
 // Group k-NN collector (k <= G). SCREEN: f64 trees measured first in FP32 from the shadow
 // against the certified bound of screen_threshold, exact FP64 only for survivors.
 template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {

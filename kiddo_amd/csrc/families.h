@@ -151,5 +151,30 @@ __global__ __launch_bounds__(kBlock) void k_within_fill(DevTree<A, K> t, const A
   flush_counters(cnt, stats);
 }
 
+// within fill, G lanes per query (dense outputs; SquaredEuclidean, the group traversal's
+// metric). The count comes from kd_within_count as for the thread form.
+template <class A, int K, int G>
+__global__ __launch_bounds__(kBlock) void k_within_fill_group(DevTree<A, K> t, const A* __restrict__ queries,
+                                                              const uint32_t* __restrict__ order, uint64_t m,
+                                                              typename Ax<A>::O radius, int excl, int sorted,
+                                                              const uint64_t* __restrict__ offsets,
+                                                              uint32_t* __restrict__ out_items,
+                                                              typename Ax<A>::O* __restrict__ out_d) {
+  using O = typename Ax<A>::O;
+  pin_stems(t);
+  const uint64_t gid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / G;
+  const int gl = int(threadIdx.x) & (G - 1);
+  if (gid >= m) return;  // uniform within a group; pin_stems synchronised the block already
+  const uint64_t qi = order ? order[gid] : gid;
+  Small<A, K> q;
+  Small<O, K> qw;
+  load_query<A, K>(queries, qi, q, qw);
+  const uint64_t o = offsets[qi], cap = offsets[qi + 1] - o;
+  GroupWithinVis<O, G> v{radius, excl != 0, gl, 0, out_items + o, out_d + o, cap};
+  Counters c;
+  group_traverse<A, K, G>(t, q, qw, v, c, gl);
+  if (sorted && gl == 0) sort_segment(out_items + o, out_d + o, v.count < cap ? v.count : cap);
+}
+
 }  // namespace dev
 }  // namespace kd

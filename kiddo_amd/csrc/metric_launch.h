@@ -303,6 +303,17 @@ int mlaunch_within_fill(const QueryArgs& a, M met, double radius, const uint64_t
     using O = typename dev::Ax<A>::O;
     auto dt = dev_tree<A, KK>(a.t);
     KernelTimer kt(a.t, a.s);
+    if constexpr (std::is_same<M, dev::SqEuclidean>::value) {
+      if (a.dense) {  // 16 lanes per query, coalesced result writes
+        constexpr int G = 16;
+        hipLaunchKernelGGL((dev::k_within_fill_group<A, KK, G>), dim3(grid_for(a.m * G)), dim3(dev::kBlock),
+                           lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, O(radius),
+                           (a.flags & KD_EXCLUSIVE) ? 1 : 0, (a.flags & KD_UNSORTED) ? 0 : 1, offsets, items,
+                           static_cast<O*>(dists));
+        KD_HIP(hipGetLastError());
+        return KD_OK;
+      }
+    }
     hipLaunchKernelGGL((dev::k_within_fill<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds_bytes_for(dt),
                        a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, O(radius),
                        (a.flags & KD_EXCLUSIVE) ? 1 : 0, (a.flags & KD_UNSORTED) ? 0 : 1, offsets, items,

@@ -89,6 +89,7 @@ struct QueryArgs {
   const uint32_t* order;  // Morton permutation or null
   uint32_t flags;
   hipStream_t s;
+  bool dense = false;  // within fill: many results per query (group-per-query fill)
 };
 
 constexpr int kMaxK = 64;

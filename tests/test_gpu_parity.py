@@ -258,3 +258,32 @@ def test_traversal_visits_match_oracle(ka, oracle_mod, cuda):
     t.set_timing(True)
     t.nearest_n_raw(q, 16)
     assert t.last_kernel_ms() > 0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_within_dense_group_fill(ka, oracle_mod, cuda, dtype):
+    """Dense within outputs (>= 32 results per query) take the group-per-query fill; its
+    results equal the oracle's (sorted: bit for bit; unsorted: the same multiset per query,
+    in the thread form's order)."""
+    rng = np.random.default_rng(21)
+    pts = rng.random((30_000, 3)).astype(dtype)
+    q = rng.random((3000, 3)).astype(dtype)
+    q[:50] = pts[:50]
+    r2 = 0.01  # about 100 entries per query (r = 0.1)
+    t = ka.ImmutableKdTree.new_from_slice(pts, 32)
+    ot = oracle_mod.Tree.build(pts, 32)
+    for excl in (False, True):
+        _announce("dense within", np.dtype(dtype).name, excl)
+        oo, oi, od = ot.within(q, r2, excl, True)
+        assert int(oo[-1]) >= 32 * len(q)
+        for flags in (0, ka.KD_MORTON):
+            f = flags | (ka.KD_EXCLUSIVE if excl else 0)
+            r = t.within_raw(q, r2, f)
+            assert_same_bits(r.offsets.cpu().numpy().astype(np.uint64), oo, "offsets")
+            assert_same_bits(r.items.cpu().numpy(), oi, "items")
+            assert_same_bits(r.distances.cpu().numpy(), od, "dists")
+            ru = t.within_raw(q, r2, f | ka.KD_UNSORTED)
+            ui, ud = ru.items.cpu().numpy(), ru.distances.cpu().numpy()
+            for i in range(0, len(q), 37):
+                s, e = int(oo[i]), int(oo[i + 1])
+                assert sorted(zip(ud[s:e].tolist(), ui[s:e].tolist())) == list(zip(od[s:e].tolist(), oi[s:e].tolist()))
