@@ -287,3 +287,24 @@ def test_within_dense_group_fill(ka, oracle_mod, cuda, dtype):
             for i in range(0, len(q), 37):
                 s, e = int(oo[i]), int(oo[i + 1])
                 assert sorted(zip(ud[s:e].tolist(), ui[s:e].tolist())) == list(zip(od[s:e].tolist(), oi[s:e].tolist()))
+
+
+@pytest.mark.parametrize("r2,dense", [(0.006, True), (0.0004, False)], ids=["dense", "sparse"])
+def test_within_large_batch(ka, oracle_mod, cuda, r2, dense):
+    """70,000 queries, dense (group-per-query fill) and sparse (thread fill) outputs:
+    offsets and sorted results equal the oracle's."""
+    rng = np.random.default_rng(23)
+    pts = rng.random((30_000, 3))
+    q = rng.random((70_000, 3))
+    q[:64] = pts[:64]
+    t = ka.ImmutableKdTree.new_from_slice(pts, 32)
+    ot = oracle_mod.Tree.build(pts, 32)
+    for excl in (False, True):
+        _announce("within large batch", r2, excl)
+        oo, oi, od = ot.within(q, r2, excl, True)
+        assert (int(oo[-1]) >= 32 * len(q)) == dense
+        for flags in (0, ka.KD_MORTON):
+            r = t.within_raw(q, r2, flags | (ka.KD_EXCLUSIVE if excl else 0))
+            assert_same_bits(r.offsets.cpu().numpy().astype(np.uint64), oo, "offsets")
+            assert_same_bits(r.items.cpu().numpy(), oi, "items")
+            assert_same_bits(r.distances.cpu().numpy(), od, "dists")
