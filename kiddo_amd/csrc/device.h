@@ -332,6 +332,11 @@ template <class O> struct NearestOneVis {
 };
 
 // Sorted top-k in registers, ordered by (distance, item). Unused slots hold (+inf, ~0u).
+// The middle of three values. For a <= c it is b clamped into [a, c]: slot j of a sorted
+// list after b is inserted, from the old slots j-1 and j. Distances here are never NaN.
+__device__ __forceinline__ float med3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+__device__ __forceinline__ double med3(double a, double b, double c) { return fmin(c, fmax(a, b)); }
+
 template <class O, int MAXK> struct TopK {
   O d[MAXK];
   uint32_t it[MAXK];
@@ -343,26 +348,25 @@ template <class O, int MAXK> struct TopK {
     }
   }
   static __device__ __forceinline__ bool less(O ad, uint32_t ai, O bd, uint32_t bi) {
-    return ad < bd || (ad == bd && ai < bi);
+    return (ad < bd) | ((ad == bd) & (ai < bi));  // bitwise: no branch per slot
   }
   // inserts c, dropping the last slot
   __device__ __forceinline__ void insert(O cd, uint32_t ci) { insert_if(true, cd, ci); }
   // inserts c when ok (a predicate rather than a branch: no copy of the slots is made)
+  // The distances shift by clamping (one med3, or a max and a min, per slot); the items
+  // follow the (distance, item) comparisons, which also place equal distances.
   __device__ __forceinline__ void insert_if(bool ok, O cd, uint32_t ci) {
     bool lt[MAXK];
 #pragma unroll
-    for (int j = 0; j < MAXK; ++j) lt[j] = ok && less(cd, ci, d[j], it[j]);
+    for (int j = 0; j < MAXK; ++j) lt[j] = ok & less(cd, ci, d[j], it[j]);
+    const O c = ok ? cd : oinf<O>();  // +inf leaves every distance slot as it is
 #pragma unroll
     for (int j = MAXK - 1; j > 0; --j) {
-      const bool sh = lt[j - 1];
-      const bool here = lt[j];
-      d[j] = sh ? d[j - 1] : (here ? cd : d[j]);
-      it[j] = sh ? it[j - 1] : (here ? ci : it[j]);
+      it[j] = lt[j - 1] ? it[j - 1] : (lt[j] ? ci : it[j]);
+      d[j] = med3(d[j - 1], c, d[j]);
     }
-    if (lt[0]) {
-      d[0] = cd;
-      it[0] = ci;
-    }
+    it[0] = lt[0] ? ci : it[0];
+    d[0] = d[0] < c ? d[0] : c;
   }
   __device__ __forceinline__ void kth(int k, O& kd, uint32_t& kit) const {
     if (k == MAXK) {
@@ -421,27 +425,44 @@ template <class O, int MAXK> struct KnnVis {
     cnt = ok ? (cnt < k ? cnt + 1 : k) : cnt;
     if (cnt >= k) top.kth(k, kd, kit);
   }
-  // The distances of kChunk entries are computed in one branch-free block (indices clamped
-  // to the leaf), so their coordinate loads are in flight together; candidates then load
-  // their item and are offered in entry order.
+  __device__ __forceinline__ bool candidate(O d) const {
+    return (excl ? d < radius : d <= radius) && d <= cap && (cnt < k || d <= kd);
+  }
+  // Two passes per leaf. The first measures the first 64 entries (4 at a time, indices
+  // clamped to the leaf, so their loads are in flight together) and marks in a bit mask
+  // those that pass the bound of the start of the leaf; the second measures the marked
+  // ones again and offers them, in entry order, against the current bound. Entries past
+  // the first 64 (long leaves of equal coordinates) are measured one at a time in the
+  // second loop. The top-k is updated at one place inside one loop: a second update site,
+  // or a loop around this one, makes the compiler hold a second copy of the top-k.
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
     constexpr uint32_t kChunk = 4;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
-    for (uint32_t p0 = s; p0 < e; p0 += kChunk) {
+    const uint32_t L = e - s, nm = L < 64u ? L : 64u;
+    uint64_t mask = 0ull;
+    for (uint32_t j0 = 0; j0 < nm; j0 += kChunk) {
       O dd[kChunk];
 #pragma unroll
-      for (uint32_t u = 0; u < kChunk; ++u) dd[u] = point_dist(t, p0 + u < e ? p0 + u : e - 1, qw);
+      for (uint32_t u = 0; u < kChunk; ++u) dd[u] = point_dist(t, s + (j0 + u < nm ? j0 + u : nm - 1), qw);
 #pragma unroll
-      for (uint32_t u = 0; u < kChunk; ++u) {
-        const uint32_t p = p0 + u;
-        const O d = dd[u];
-        const bool cand = p < e && (excl ? d < radius : d <= radius) && d <= cap && (cnt < k || d <= kd);
-        if (cand) {
-          const uint32_t it = t.items[p];
-          offer_if(cnt < k || TopK<O, MAXK>::less(d, it, kd, kit), d, it);
-        }
+      for (uint32_t u = 0; u < kChunk; ++u)
+        mask |= uint64_t((j0 + u < nm) && candidate(dd[u]) ? 1u : 0u) << (j0 + u);
+    }
+    uint32_t next = nm;  // first entry not covered by the mask
+    for (;;) {
+      if (mask == 0ull && next >= L) break;
+      uint32_t j;
+      if (mask != 0ull) {
+        j = uint32_t(__ffsll((unsigned long long)mask) - 1);
+        mask &= mask - 1ull;
+      } else {
+        j = next++;
       }
+      const uint32_t p = s + j;
+      const O d = point_dist(t, p, qw);
+      const uint32_t it = t.items[p];
+      offer_if(candidate(d) && (cnt < k || TopK<O, MAXK>::less(d, it, kd, kit)), d, it);
     }
   }
 };
@@ -566,52 +587,75 @@ template <int MAXK> struct ScreenKnnVis {
   __device__ double bound() const { return v.bound(); }
   __device__ bool prune(double rd) const { return v.prune(rd); }
   __device__ bool unbounded() const { return v.unbounded(); }
-  // Two passes per run of up to 32 entries: the FP32 screen marks survivors in a bit mask
-  // against the threshold at the start of the run, then only survivors are measured in
-  // FP64 and offered (in entry order) to the collector. The wave then pays the FP64 and
-  // insertion work once per survivor of its busiest lane, not once per scanned entry.
-  // The screen issues the shadow loads of 8 entries before using any (indices clamped to
-  // the leaf, so every load is in bounds), a survivor's coordinates and item are loaded
-  // together, and the insertion is predicated rather than branched.
+  template <int K>
+  __device__ __forceinline__ float screen_dist(const DevTree<double, K>& t, uint32_t p) const {
+    const float4 sh = t.shadow[p];
+    const float dx = sub_rn(sh.x, qf[0]);
+    float acc = mul_rn(dx, dx);
+    const float dy = sub_rn(sh.y, qf[1]);
+    acc = add_rn(acc, mul_rn(dy, dy));
+    if (K > 2) {
+      const float dz = sub_rn(sh.z, qf[2]);
+      acc = add_rn(acc, mul_rn(dz, dz));
+    }
+    return acc;
+  }
+  // Two passes per leaf. The FP32 screen marks the survivors among the first 64 entries in
+  // a bit mask against the threshold at the start of the leaf (shadow loads issued 4 at a
+  // time, indices clamped to the leaf); then only survivors are measured in FP64 and
+  // offered, in entry order, to the collector. Entries past the first 64 (long leaves of
+  // equal coordinates) are screened one at a time in the same loop. The top-k is updated
+  // at one place inside one loop: a second update site, or a loop around this one, makes
+  // the compiler hold a second copy of the top-k in registers.
   template <int K, class QW>
   __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf) {
     constexpr uint32_t kBatch = 4;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
-    for (uint32_t c0 = s; c0 < e; c0 += 32) {
-      const uint32_t n = (e - c0) < 32u ? (e - c0) : 32u;
-      // unbounded (top-k not yet full, no radius): every entry survives the screen
-      const bool screen = thr < __builtin_huge_valf();
-      uint32_t mask = screen ? 0u : (n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u);
-      for (uint32_t j0 = 0; screen && j0 < n; j0 += kBatch) {
-        float4 sh[kBatch];
+    const uint32_t L = e - s, nm = L < 64u ? L : 64u;
+    // unbounded (top-k not yet full, no radius): every entry survives the screen
+    const bool screen = thr < __builtin_huge_valf();
+    uint64_t mask = screen ? 0ull : (nm == 64u ? ~0ull : (1ull << nm) - 1ull);
+    for (uint32_t j0 = 0; screen && j0 < nm; j0 += kBatch) {
+      float4 sh[kBatch];
 #pragma unroll
-        for (uint32_t u = 0; u < kBatch; ++u) sh[u] = t.shadow[c0 + (j0 + u < n ? j0 + u : n - 1)];
+      for (uint32_t u = 0; u < kBatch; ++u) sh[u] = t.shadow[s + (j0 + u < nm ? j0 + u : nm - 1)];
 #pragma unroll
-        for (uint32_t u = 0; u < kBatch; ++u) {
-          const float dx = sub_rn(sh[u].x, qf[0]);
-          float acc = mul_rn(dx, dx);
-          const float dy = sub_rn(sh[u].y, qf[1]);
-          acc = add_rn(acc, mul_rn(dy, dy));
-          if (K > 2) {
-            const float dz = sub_rn(sh[u].z, qf[2]);
-            acc = add_rn(acc, mul_rn(dz, dz));
-          }
-          mask |= ((j0 + u < n) && !(acc > thr) ? 1u : 0u) << (j0 + u);
+      for (uint32_t u = 0; u < kBatch; ++u) {
+        const float dx = sub_rn(sh[u].x, qf[0]);
+        float acc = mul_rn(dx, dx);
+        const float dy = sub_rn(sh[u].y, qf[1]);
+        acc = add_rn(acc, mul_rn(dy, dy));
+        if (K > 2) {
+          const float dz = sub_rn(sh[u].z, qf[2]);
+          acc = add_rn(acc, mul_rn(dz, dz));
         }
-      }
-      while (mask) {
-        const uint32_t j = uint32_t(__ffs(mask) - 1);
-        mask &= mask - 1u;
-        const uint32_t p = c0 + j;
-        ++rescored;
-        const uint32_t it = t.items[p];
-        const double d = point_dist(t, p, qw);
-        bool ok = (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
-        ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
-        v.offer_if(ok, d, it);
-        thr = ok ? screen_threshold(v.bound(), eps_k) : thr;
+        mask |= uint64_t((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << (j0 + u);
       }
     }
+    uint32_t next = nm;  // first entry not covered by the mask
+    for (;;) {
+      if (mask == 0ull && next >= L) break;
+      uint32_t j;
+      bool cand = true;
+      if (mask != 0ull) {
+        j = uint32_t(__ffsll((unsigned long long)mask) - 1);
+        mask &= mask - 1ull;
+      } else {
+        j = next++;
+        cand = !(screen_dist(t, s + j) > thr);
+      }
+      const uint32_t p = s + j;
+      rescored += cand ? 1u : 0u;
+      const uint32_t it = t.items[p];
+      const double d = point_dist(t, p, qw);
+      bool ok = cand && (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
+      ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
+      v.offer_if(ok, d, it);
+      if (j + 1u >= 64u) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
+    }
+    // The bound only tightens, so a threshold from the start of the leaf stays valid for
+    // the mask; refresh it once for the next leaf.
+    thr = screen_threshold(v.bound(), eps_k);
   }
 };
 

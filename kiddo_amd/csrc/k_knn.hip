@@ -81,12 +81,9 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
 #pragma unroll
     for (uint32_t u = 0; u < kBatch; ++u) {
       const float c = d32[u];
-      bool lt[MAXK];
 #pragma unroll
-      for (int j = 0; j < MAXK; ++j) lt[j] = c < top[j];
-#pragma unroll
-      for (int j = MAXK - 1; j > 0; --j) top[j] = lt[j - 1] ? top[j - 1] : (lt[j] ? c : top[j]);
-      top[0] = lt[0] ? c : top[0];
+      for (int j = MAXK - 1; j > 0; --j) top[j] = med3(top[j - 1], c, top[j]);  // sorted insert
+      top[0] = fminf(top[0], c);
     }
   }
   float kth = top[0];
