@@ -428,20 +428,43 @@ template <class O, int MAXK> struct KnnVis {
   __device__ __forceinline__ bool candidate(O d) const {
     return (excl ? d < radius : d <= radius) && d <= cap && (cnt < k || d <= kd);
   }
-  // Two passes per leaf. The first measures the first 64 entries (4 at a time, indices
-  // clamped to the leaf, so their loads are in flight together) and marks in a bit mask
-  // those that pass the bound of the start of the leaf; the second measures the marked
-  // ones again and offers them, in entry order, against the current bound. Entries past
-  // the first 64 (long leaves of equal coordinates) are measured one at a time in the
-  // second loop. The top-k is updated at one place inside one loop: a second update site,
-  // or a loop around this one, makes the compiler hold a second copy of the top-k.
+  // MAXK <= 4: one pass, the distances of 4 entries in one branch-free block (indices
+  // clamped to the leaf, so their loads are in flight together), candidates offered in
+  // entry order. The small top-k costs few registers even when the compiler keeps a
+  // second copy of it for the four update sites.
+  //
+  // MAXK >= 8: two passes. The first measures the first 64 entries (4 at a time) and marks
+  // in a bit mask those that pass the bound of the start of the leaf (all of them,
+  // unmeasured, while the bound is +inf); the second measures the marked ones again and
+  // offers them, in entry order, against the current bound. Entries past the first 64
+  // (long leaves of equal coordinates) follow in the same loop. The top-k is updated at
+  // one place inside one loop: a second update site, or a loop around this one, makes the
+  // compiler hold a second copy of the top-k (+3 registers per slot).
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
     constexpr uint32_t kChunk = 4;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    if constexpr (MAXK <= 4) {
+      for (uint32_t p0 = s; p0 < e; p0 += kChunk) {
+        O dd[kChunk];
+#pragma unroll
+        for (uint32_t u = 0; u < kChunk; ++u) dd[u] = point_dist(t, p0 + u < e ? p0 + u : e - 1, qw);
+#pragma unroll
+        for (uint32_t u = 0; u < kChunk; ++u) {
+          const uint32_t p = p0 + u;
+          if (p < e && candidate(dd[u])) {
+            const uint32_t it = t.items[p];
+            offer_if(cnt < k || TopK<O, MAXK>::less(dd[u], it, kd, kit), dd[u], it);
+          }
+        }
+      }
+      return;
+    }
     const uint32_t L = e - s, nm = L < 64u ? L : 64u;
-    uint64_t mask = 0ull;
-    for (uint32_t j0 = 0; j0 < nm; j0 += kChunk) {
+    // unbounded (top-k not yet full, no radius): every entry is a candidate, unmeasured
+    const bool bounded = bound() < oinf<O>();
+    uint64_t mask = bounded ? 0ull : (nm == 64u ? ~0ull : (1ull << nm) - 1ull);
+    for (uint32_t j0 = 0; bounded && j0 < nm; j0 += kChunk) {
       O dd[kChunk];
 #pragma unroll
       for (uint32_t u = 0; u < kChunk; ++u) dd[u] = point_dist(t, s + (j0 + u < nm ? j0 + u : nm - 1), qw);
@@ -450,18 +473,23 @@ template <class O, int MAXK> struct KnnVis {
         mask |= uint64_t((j0 + u < nm) && candidate(dd[u]) ? 1u : 0u) << (j0 + u);
     }
     uint32_t next = nm;  // first entry not covered by the mask
-    for (;;) {
-      if (mask == 0ull && next >= L) break;
-      uint32_t j;
-      if (mask != 0ull) {
-        j = uint32_t(__ffsll((unsigned long long)mask) - 1);
-        mask &= mask - 1ull;
-      } else {
-        j = next++;
-      }
-      const uint32_t p = s + j;
-      const O d = point_dist(t, p, qw);
-      const uint32_t it = t.items[p];
+    // one entry ahead: the loads of the next entry are issued before the current one is
+    // offered (every step is branch-free, so they can overlap the insertion)
+    bool have = mask != 0ull || next < L;
+    uint32_t j = mask != 0ull ? uint32_t(__ffsll((unsigned long long)mask) - 1) : next;
+    next += (mask == 0ull && have) ? 1u : 0u;
+    mask &= mask - 1ull;
+    O dn = have ? point_dist(t, s + j, qw) : O(0);
+    uint32_t in = have ? t.items[s + j] : 0u;
+    while (have) {
+      const O d = dn;
+      const uint32_t it = in;
+      have = mask != 0ull || next < L;
+      j = mask != 0ull ? uint32_t(__ffsll((unsigned long long)mask) - 1) : (next < L ? next : L - 1);
+      next += (mask == 0ull && have) ? 1u : 0u;
+      mask &= mask - 1ull;
+      dn = point_dist(t, s + j, qw);  // in bounds: j < L (a repeat of the last entry when done)
+      in = t.items[s + j];
       offer_if(candidate(d) && (cnt < k || TopK<O, MAXK>::less(d, it, kd, kit)), d, it);
     }
   }
@@ -633,25 +661,36 @@ template <int MAXK> struct ScreenKnnVis {
       }
     }
     uint32_t next = nm;  // first entry not covered by the mask
-    for (;;) {
-      if (mask == 0ull && next >= L) break;
-      uint32_t j;
-      bool cand = true;
-      if (mask != 0ull) {
-        j = uint32_t(__ffsll((unsigned long long)mask) - 1);
-        mask &= mask - 1ull;
-      } else {
-        j = next++;
-        cand = !(screen_dist(t, s + j) > thr);
-      }
-      const uint32_t p = s + j;
+    // one survivor ahead: its loads are issued before the current one is offered (every
+    // step is branch-free, so they can overlap the insertion)
+    bool have = mask != 0ull || next < L;
+    bool fromtail = mask == 0ull && have;
+    uint32_t j = mask != 0ull ? uint32_t(__ffsll((unsigned long long)mask) - 1) : next;
+    next += fromtail ? 1u : 0u;
+    mask &= mask - 1ull;
+    double dn = have ? point_dist(t, s + j, qw) : 0.0;
+    uint32_t in = have ? t.items[s + j] : 0u;
+    bool cn = have && (!fromtail || !(screen_dist(t, s + j) > thr));
+    uint32_t jc = j;
+    while (have) {
+      const double d = dn;
+      const uint32_t it = in;
+      const bool cand = cn;
+      const uint32_t jcur = jc;
+      have = mask != 0ull || next < L;
+      fromtail = mask == 0ull && have;
+      j = mask != 0ull ? uint32_t(__ffsll((unsigned long long)mask) - 1) : (next < L ? next : L - 1);
+      next += fromtail ? 1u : 0u;
+      mask &= mask - 1ull;
+      dn = point_dist(t, s + j, qw);  // in bounds: j < L (a repeat of the last entry when done)
+      in = t.items[s + j];
+      cn = have && (!fromtail || !(screen_dist(t, s + j) > thr));
+      jc = j;
       rescored += cand ? 1u : 0u;
-      const uint32_t it = t.items[p];
-      const double d = point_dist(t, p, qw);
       bool ok = cand && (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
       ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
       v.offer_if(ok, d, it);
-      if (j + 1u >= 64u) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
+      if (jcur + 1u >= 64u) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
     }
     // The bound only tightens, so a threshold from the start of the leaf stays valid for
     // the mask; refresh it once for the next leaf.
