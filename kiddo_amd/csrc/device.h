@@ -211,10 +211,15 @@ struct Counters {
 };
 
 // ------------------------------------------------------------------ traversal --------
-// V supplies bound(), prune(rd), unbounded() and leaf(tree, qw, leaf_index).
+// V supplies bound(), prune(rd), unbounded() and leaf(tree, qw, first_leaf, n_leaves), which
+// scans the entries of n_leaves consecutive leaves (one, except for the first visit below).
+// first_up > 0: the first descent stops 2^first_up leaves above the bottom and that whole
+// block of leaves (the query's own) is scanned as one visit; its subtree is not entered
+// again. Any visit order finds the same canonical result (the bound only prunes what cannot
+// reach it); Kiddo's own order is first_up = 0.
 template <class A, int K, class V, class QW>
 __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, K>& q, const QW& qw, V& vis,
-                                         Counters& c) {
+                                         Counters& c, int first_up = 0) {
   // Register-only form of Kiddo's DFS: no per-level stack. A queued far child at level l is
   // a `pending` bit; its rect distance rd' = (rd - off[dim]^2) + new_off^2 is recomputed
   // when the bit is popped, from the same rd and off[] it was computed from at push time
@@ -233,10 +238,11 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
   uint32_t idx = 1;
   int level = 0, dim = 0;
   bool check = !vis.unbounded();
+  int lim = D - (first_up < D ? first_up : D);
   for (;;) {
     // descend to a leaf (orchestrator/mod.rs:520-603)
-    c.stems += uint32_t(D - level);
-    while (level < D) {
+    c.stems += uint32_t(lim - level);
+    while (level < lim) {
       const A pivot = stem_at(t, idx);
       if (pivot < Ax<A>::maxv()) {
         const bool right = q[dim] >= pivot;
@@ -255,10 +261,14 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
       dim = (dim + 1 == K) ? 0 : dim + 1;
     }
     {
+      const uint32_t nl = 1u << (D - lim);
+      idx <<= (D - lim);  // the block's first leaf: its ancestors above the block are unchanged
+      level = D;
+      lim = D;
       const uint32_t leaf = idx - (1u << D);
-      c.leaves += 1;
-      c.points += t.leaf_start[leaf + 1] - t.leaf_start[leaf];
-      vis.leaf(t, qw, leaf);
+      c.leaves += nl;
+      c.points += t.leaf_start[leaf + nl] - t.leaf_start[leaf];
+      vis.leaf(t, qw, leaf, nl);
     }
     // pop frames from the deepest level upwards (orchestrator/mod.rs:450-515)
     bool entered = false;
@@ -316,8 +326,8 @@ template <class O> struct NearestOneVis {
   __device__ bool prune(O rd) const { return rd > best; }
   __device__ bool unbounded() const { return true; }
   template <class A, int K, class QW>
-  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
-    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
     for (uint32_t p = s; p < e; ++p) {
       const O d = point_dist(t, p, qw);
       if (d <= best) {
@@ -441,9 +451,9 @@ template <class O, int MAXK> struct KnnVis {
   // one place inside one loop: a second update site, or a loop around this one, makes the
   // compiler hold a second copy of the top-k (+3 registers per slot).
   template <class A, int K, class QW>
-  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
     constexpr uint32_t kChunk = 4;
-    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
     if constexpr (MAXK <= 4) {
       for (uint32_t p0 = s; p0 < e; p0 += kChunk) {
         O dd[kChunk];
@@ -508,8 +518,8 @@ template <class O> struct WithinVis {
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
   template <class A, int K, class QW>
-  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
-    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
     for (uint32_t p = s; p < e; ++p) {
       const O d = point_dist(t, p, qw);
       const bool ok = excl ? d < radius : d <= radius;
@@ -548,8 +558,8 @@ template <class O, int MAXK> struct BestNVis {
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
   template <class A, int K, class QW>
-  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
-    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
     for (uint32_t p = s; p < e; ++p) {
       const O dd = point_dist(t, p, qw);
       const bool ok = excl ? dd < radius : dd <= radius;
@@ -628,25 +638,33 @@ template <int MAXK> struct ScreenKnnVis {
     }
     return acc;
   }
-  // Two passes per leaf. The FP32 screen marks the survivors among the first 64 entries in
-  // a bit mask against the threshold at the start of the leaf (shadow loads issued 4 at a
-  // time, indices clamped to the leaf); then only survivors are measured in FP64 and
-  // offered, in entry order, to the collector. Entries past the first 64 (long leaves of
-  // equal coordinates) are screened one at a time in the same loop. The top-k is updated
-  // at one place inside one loop: a second update site, or a loop around this one, makes
-  // the compiler hold a second copy of the top-k in registers.
+  // Two passes per visit. The FP32 screen marks the survivors among the first 128 entries (64
+  // below a top-k of 16: its first visit, the pre-scan block, is smaller) in two 64-bit masks
+  // against the threshold at the start of the visit (shadow loads issued 4
+  // at a time, indices clamped to the visit); then only survivors are measured in FP64 and
+  // offered, in entry order, to the collector, one per step, the next one's loads issued a
+  // step ahead. Entries past the first 128 (long leaves of equal coordinates) are screened
+  // one at a time in the same loop. The top-k is updated at one place inside one loop: a
+  // second update site, or a loop around this one, makes the compiler hold a second copy of
+  // the top-k in registers.
   template <int K, class QW>
-  __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf) {
+  __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
     constexpr uint32_t kBatch = 4;
-    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
-    const uint32_t L = e - s, nm = L < 64u ? L : 64u;
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
+    constexpr uint32_t kMaskN = MAXK >= 16 ? 128u : 64u;
+    const uint32_t L = e - s, nm = L < kMaskN ? L : kMaskN;
     // unbounded (top-k not yet full, no radius): every entry survives the screen
     const bool screen = thr < __builtin_huge_valf();
-    uint64_t mask = screen ? 0ull : (nm == 64u ? ~0ull : (1ull << nm) - 1ull);
+    uint64_t m0 = 0ull, m1 = 0ull;
+    if (!screen) {
+      m0 = nm >= 64u ? ~0ull : (1ull << nm) - 1ull;
+      m1 = nm >= 128u ? ~0ull : nm > 64u ? (1ull << (nm - 64u)) - 1ull : 0ull;
+    }
     for (uint32_t j0 = 0; screen && j0 < nm; j0 += kBatch) {
       float4 sh[kBatch];
 #pragma unroll
       for (uint32_t u = 0; u < kBatch; ++u) sh[u] = t.shadow[s + (j0 + u < nm ? j0 + u : nm - 1)];
+      uint32_t bits = 0;
 #pragma unroll
       for (uint32_t u = 0; u < kBatch; ++u) {
         const float dx = sub_rn(sh[u].x, qf[0]);
@@ -657,17 +675,21 @@ template <int MAXK> struct ScreenKnnVis {
           const float dz = sub_rn(sh[u].z, qf[2]);
           acc = add_rn(acc, mul_rn(dz, dz));
         }
-        mask |= uint64_t((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << (j0 + u);
+        bits |= ((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << u;
       }
+      // j0 is a multiple of 4, so the 4 bits never straddle the two words
+      if (j0 < 64u) m0 |= uint64_t(bits) << j0;
+      else m1 |= uint64_t(bits) << (j0 - 64u);
     }
-    uint32_t next = nm;  // first entry not covered by the mask
+    uint32_t next = nm;  // first entry not covered by the masks
     // one survivor ahead: its loads are issued before the current one is offered (every
     // step is branch-free, so they can overlap the insertion)
-    bool have = mask != 0ull || next < L;
-    bool fromtail = mask == 0ull && have;
-    uint32_t j = mask != 0ull ? uint32_t(__ffsll((unsigned long long)mask) - 1) : next;
+    bool have = (m0 | m1) != 0ull || next < L;
+    bool fromtail = (m0 | m1) == 0ull && have;
+    uint32_t j = m0 != 0ull ? uint32_t(__ffsll((unsigned long long)m0) - 1)
+                 : m1 != 0ull ? 64u + uint32_t(__ffsll((unsigned long long)m1) - 1) : next;
     next += fromtail ? 1u : 0u;
-    mask &= mask - 1ull;
+    if (m0 != 0ull) m0 &= m0 - 1ull; else m1 &= m1 - 1ull;
     double dn = have ? point_dist(t, s + j, qw) : 0.0;
     uint32_t in = have ? t.items[s + j] : 0u;
     bool cn = have && (!fromtail || !(screen_dist(t, s + j) > thr));
@@ -677,11 +699,12 @@ template <int MAXK> struct ScreenKnnVis {
       const uint32_t it = in;
       const bool cand = cn;
       const uint32_t jcur = jc;
-      have = mask != 0ull || next < L;
-      fromtail = mask == 0ull && have;
-      j = mask != 0ull ? uint32_t(__ffsll((unsigned long long)mask) - 1) : (next < L ? next : L - 1);
+      have = (m0 | m1) != 0ull || next < L;
+      fromtail = (m0 | m1) == 0ull && have;
+      j = m0 != 0ull ? uint32_t(__ffsll((unsigned long long)m0) - 1)
+          : m1 != 0ull ? 64u + uint32_t(__ffsll((unsigned long long)m1) - 1) : (next < L ? next : L - 1);
       next += fromtail ? 1u : 0u;
-      mask &= mask - 1ull;
+      if (m0 != 0ull) m0 &= m0 - 1ull; else m1 &= m1 - 1ull;
       dn = point_dist(t, s + j, qw);  // in bounds: j < L (a repeat of the last entry when done)
       in = t.items[s + j];
       cn = have && (!fromtail || !(screen_dist(t, s + j) > thr));
@@ -690,10 +713,10 @@ template <int MAXK> struct ScreenKnnVis {
       bool ok = cand && (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
       ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
       v.offer_if(ok, d, it);
-      if (jcur + 1u >= 64u) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
+      if (jcur + 1u >= kMaskN) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
     }
-    // The bound only tightens, so a threshold from the start of the leaf stays valid for
-    // the mask; refresh it once for the next leaf.
+    // The bound only tightens, so a threshold from the start of the visit stays valid for
+    // the masks; refresh it once for the next visit.
     thr = screen_threshold(v.bound(), eps_k);
   }
 };

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_approx_nearest_one(DevTree<A, K> t, 
       cnt.stems += uint32_t(t.nlev);
       cnt.leaves += 1;
       cnt.points += t.leaf_start[leaf + 1] - t.leaf_start[leaf];
-      v.leaf(t, mq, leaf);
+      v.leaf(t, mq, leaf, 1u);
     }
     out_items[qi] = v.item;
     out_d[qi] = A(v.best);

@@ -37,10 +37,12 @@ __device__ __forceinline__ void screen_setup(const Small<double, K>& q, const do
 // (sqrt(S32 (1+4u)) + sqrt(K) eps)^2 (1 + 2e-15): the FP32 sum of non-negative squares is
 // at least ||delta||^2 (1-u)^K, each FP32 coordinate difference is within eps of the
 // exact one, and the FP64 evaluation is within (1 + 1e-15) of the exact square. Returns
-// +inf when the tree holds fewer than k entries.
+// +inf when the tree holds fewer than k entries. `up` receives the block's height in
+// levels (2^up leaves), 0 with an infinite cap.
 template <int K, int MAXK>
 __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K>& q, const float (&qf)[3],
-                              double eps_k, int k) {
+                              double eps_k, int k, int& up_out) {
+  up_out = 0;
   if (t.n < uint32_t(k)) return __builtin_huge_val();
   const int D = t.nlev;
   uint32_t idx = 1;
@@ -52,7 +54,8 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
   }
   const uint32_t leaf = idx - (1u << D);
   uint32_t b = 0, e = 0;
-  for (int up = 0;; ++up) {
+  int up = 0;
+  for (;; ++up) {
     const uint32_t first = (leaf >> up) << up;
     b = t.leaf_start[first];
     e = t.leaf_start[first + (1u << up)];
@@ -90,6 +93,7 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
 #pragma unroll
   for (int j = 1; j < MAXK; ++j) kth = (j == k - 1) ? top[j] : kth;
   if (!(kth < __builtin_huge_valf())) return __builtin_huge_val();
+  up_out = up;
   const double r = sqrt(double(kth) * (1.0 + 4.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
   return r * r * (1.0 + 2e-15);
 }
@@ -106,10 +110,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     ScreenKnnVis<MAXK> v;
     v.v.init(k, radius, excl != 0);
     screen_setup<K>(q, max_abs, v.qf, v.eps_k);
-    v.v.cap = prescan_cap<K, MAXK>(t, q, v.qf, v.eps_k, k);
+    // the pre-scan's block is then the first visit: its entries within the cap are the first
+    // candidates of every lane, and its leaves are not visited again
+    int up = 0;
+    v.v.cap = prescan_cap<K, MAXK>(t, q, v.qf, v.eps_k, k, up);
     v.thr = screen_threshold(v.v.bound(), v.eps_k);
     v.rescored = 0;
-    traverse(t, q, qw, v, cnt);
+    traverse(t, q, qw, v, cnt, up);
     uint32_t* ri = out_items + qi * uint64_t(k);
     double* rdd = out_d + qi * uint64_t(k);
 #pragma unroll

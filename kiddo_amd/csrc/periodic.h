@@ -121,7 +121,7 @@ __device__ __forceinline__ void periodic_traverse(const DevTree<A, K>& t, const 
       const uint32_t leaf = idx - (1u << D);
       c.leaves += 1;
       c.points += t.leaf_start[leaf + 1] - t.leaf_start[leaf];
-      vis.leaf(t, pq, leaf);
+      vis.leaf(t, pq, leaf, 1u);
     }
     if (!pending) return;
     // the deepest queued far child: the sibling of the current path's level-(l+1) node

@@ -230,8 +230,10 @@ def test_builder_surface(ka, cuda):
 def test_traversal_visits_match_oracle(ka, oracle_mod, cuda):
     """The plain and group kernels visit exactly the leaves (and scan exactly the entries)
     that the oracle's restatement of Kiddo's DFS visits for the same queries. The
-    thread-per-query FP32-screened kernel starts its DFS from a certified bound on the k-th
-    distance, so it visits at most those leaves (its results are checked elsewhere)."""
+    thread-per-query FP32-screened kernel first scans the query's pre-scan block (the 4
+    leaves under one stem here) and then runs the DFS from a certified bound on the k-th
+    distance, so it visits at most 3 leaves per query more than Kiddo's DFS (its results
+    are checked elsewhere)."""
     _announce("visit counters")
     rng = np.random.default_rng(12)
     pts = rng.random((100_000, 3))
@@ -245,8 +247,8 @@ def test_traversal_visits_match_oracle(ka, oracle_mod, cuda):
             t.nearest_n_raw(q, 16, flags=flags)
             st = t.query_stats()
             if variant == 1 and flags & ka.KD_SCREEN_F32:
-                assert 0 < st["leaves"] <= int(ost[0]), (variant, flags, st, ost)
-                assert 0 < st["points"] <= int(ost[1]), (variant, flags, st, ost)
+                assert 0 < st["leaves"] <= int(ost[0]) + 3 * len(q), (variant, flags, st, ost)
+                assert 0 < st["points"] <= int(ost[1]) + 3 * 32 * len(q), (variant, flags, st, ost)
                 continue
             assert st["leaves"] == int(ost[0]), (variant, flags, st, ost)
             assert st["points"] == int(ost[1]), (variant, flags, st, ost)
