@@ -616,6 +616,46 @@ __device__ __forceinline__ float screen_threshold(double T, double eps_k) {
   return f;
 }
 
+// A bit set of W*64 entries of a visit, taken lowest first (constant word indices only, so
+// it stays in registers).
+template <int W> struct EntryMask {
+  uint64_t m[W];
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int w = 0; w < W; ++w) m[w] = 0ull;
+  }
+  __device__ __forceinline__ void set_first(uint32_t n) {  // entries [0, n), n <= 64 W
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint32_t lo = 64u * uint32_t(w);
+      m[w] = n >= lo + 64u ? ~0ull : n > lo ? (1ull << (n - lo)) - 1ull : 0ull;
+    }
+  }
+  __device__ __forceinline__ void set4(uint32_t j0, uint32_t bits) {  // j0 % 4 == 0
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (j0 >= 64u * uint32_t(w) && j0 < 64u * uint32_t(w + 1)) m[w] |= uint64_t(bits) << (j0 - 64u * uint32_t(w));
+  }
+  __device__ __forceinline__ bool any() const {
+    uint64_t a = 0ull;
+#pragma unroll
+    for (int w = 0; w < W; ++w) a |= m[w];
+    return a != 0ull;
+  }
+  __device__ __forceinline__ uint32_t pop() {  // requires any()
+    uint32_t j = 0;
+    bool found = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const bool here = !found && m[w] != 0ull;
+      j = here ? 64u * uint32_t(w) + uint32_t(__ffsll((unsigned long long)m[w]) - 1) : j;
+      m[w] = here ? m[w] & (m[w] - 1ull) : m[w];
+      found = found || here;
+    }
+    return j;
+  }
+};
+
 template <int MAXK> struct ScreenKnnVis {
   KnnVis<double, MAXK> v;
   float qf[3];
@@ -638,28 +678,35 @@ template <int MAXK> struct ScreenKnnVis {
     }
     return acc;
   }
+  // The next survivor at or after entry `next` past the mask (entries screened one at a
+  // time; long leaves of equal coordinates, or the tail of a large first visit): its index,
+  // or L when there is none.
+  template <int K>
+  __device__ __forceinline__ uint32_t next_tail(const DevTree<double, K>& t, uint32_t s, uint32_t next,
+                                                uint32_t L) const {
+    while (next < L && screen_dist(t, s + next) > thr) ++next;
+    return next;
+  }
   // Two passes per visit. The FP32 screen marks the survivors among the first 128 entries (64
-  // below a top-k of 16: its first visit, the pre-scan block, is smaller) in two 64-bit masks
-  // against the threshold at the start of the visit (shadow loads issued 4
-  // at a time, indices clamped to the visit); then only survivors are measured in FP64 and
-  // offered, in entry order, to the collector, one per step, the next one's loads issued a
-  // step ahead. Entries past the first 128 (long leaves of equal coordinates) are screened
-  // one at a time in the same loop. The top-k is updated at one place inside one loop: a
-  // second update site, or a loop around this one, makes the compiler hold a second copy of
-  // the top-k in registers.
+  // below a top-k of 16, whose first visit, the pre-scan block, is smaller) in a bit mask
+  // against the threshold at the start of the visit (shadow loads issued 4 at a time,
+  // indices clamped to the visit); then only survivors are measured in FP64 and offered, in
+  // entry order, to the collector, one per step, the next one's loads issued a step ahead.
+  // Entries past the mask are screened one at a time when the mask is empty. The top-k is
+  // updated at one place inside one loop: a second update site, or a loop around this one,
+  // makes the compiler hold a second copy of the top-k in registers.
   template <int K, class QW>
   __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
     constexpr uint32_t kBatch = 4;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
-    constexpr uint32_t kMaskN = MAXK >= 16 ? 128u : 64u;
+    constexpr int kWords = MAXK >= 16 ? 2 : 1;
+    constexpr uint32_t kMaskN = 64u * kWords;
     const uint32_t L = e - s, nm = L < kMaskN ? L : kMaskN;
     // unbounded (top-k not yet full, no radius): every entry survives the screen
     const bool screen = thr < __builtin_huge_valf();
-    uint64_t m0 = 0ull, m1 = 0ull;
-    if (!screen) {
-      m0 = nm >= 64u ? ~0ull : (1ull << nm) - 1ull;
-      m1 = nm >= 128u ? ~0ull : nm > 64u ? (1ull << (nm - 64u)) - 1ull : 0ull;
-    }
+    EntryMask<kWords> mk;
+    mk.clear();
+    if (!screen) mk.set_first(nm);
     for (uint32_t j0 = 0; screen && j0 < nm; j0 += kBatch) {
       float4 sh[kBatch];
 #pragma unroll
@@ -677,46 +724,37 @@ template <int MAXK> struct ScreenKnnVis {
         }
         bits |= ((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << u;
       }
-      // j0 is a multiple of 4, so the 4 bits never straddle the two words
-      if (j0 < 64u) m0 |= uint64_t(bits) << j0;
-      else m1 |= uint64_t(bits) << (j0 - 64u);
+      mk.set4(j0, bits);  // j0 is a multiple of 4: the 4 bits never straddle two words
     }
-    uint32_t next = nm;  // first entry not covered by the masks
+    uint32_t next = nm;  // first entry not covered by the mask
     // one survivor ahead: its loads are issued before the current one is offered (every
-    // step is branch-free, so they can overlap the insertion)
-    bool have = (m0 | m1) != 0ull || next < L;
-    bool fromtail = (m0 | m1) == 0ull && have;
-    uint32_t j = m0 != 0ull ? uint32_t(__ffsll((unsigned long long)m0) - 1)
-                 : m1 != 0ull ? 64u + uint32_t(__ffsll((unsigned long long)m1) - 1) : next;
-    next += fromtail ? 1u : 0u;
-    if (m0 != 0ull) m0 &= m0 - 1ull; else m1 &= m1 - 1ull;
+    // step is branch-free apart from the tail screen, so they can overlap the insertion)
+    bool inmask = mk.any();
+    if (!inmask) next = next_tail(t, s, next, L);
+    bool have = inmask || next < L;
+    uint32_t j = inmask ? mk.pop() : next;
+    next += (!inmask && have) ? 1u : 0u;
     double dn = have ? point_dist(t, s + j, qw) : 0.0;
     uint32_t in = have ? t.items[s + j] : 0u;
-    bool cn = have && (!fromtail || !(screen_dist(t, s + j) > thr));
-    uint32_t jc = j;
     while (have) {
       const double d = dn;
       const uint32_t it = in;
-      const bool cand = cn;
-      const uint32_t jcur = jc;
-      have = (m0 | m1) != 0ull || next < L;
-      fromtail = (m0 | m1) == 0ull && have;
-      j = m0 != 0ull ? uint32_t(__ffsll((unsigned long long)m0) - 1)
-          : m1 != 0ull ? 64u + uint32_t(__ffsll((unsigned long long)m1) - 1) : (next < L ? next : L - 1);
-      next += fromtail ? 1u : 0u;
-      if (m0 != 0ull) m0 &= m0 - 1ull; else m1 &= m1 - 1ull;
+      const uint32_t jcur = j;
+      inmask = mk.any();
+      if (!inmask) next = next_tail(t, s, next, L);
+      have = inmask || next < L;
+      j = inmask ? mk.pop() : (next < L ? next : L - 1);
+      next += (!inmask && have) ? 1u : 0u;
       dn = point_dist(t, s + j, qw);  // in bounds: j < L (a repeat of the last entry when done)
       in = t.items[s + j];
-      cn = have && (!fromtail || !(screen_dist(t, s + j) > thr));
-      jc = j;
-      rescored += cand ? 1u : 0u;
-      bool ok = cand && (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
+      ++rescored;
+      bool ok = (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
       ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
       v.offer_if(ok, d, it);
       if (jcur + 1u >= kMaskN) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
     }
     // The bound only tightens, so a threshold from the start of the visit stays valid for
-    // the masks; refresh it once for the next visit.
+    // the mask; refresh it once for the next visit.
     thr = screen_threshold(v.bound(), eps_k);
   }
 };
