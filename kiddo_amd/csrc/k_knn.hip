@@ -32,8 +32,9 @@ __device__ __forceinline__ void screen_setup(const Small<double, K>& q, const do
 
 // Certified upper bound on the k-th smallest FP64 distance from q, found before the
 // traversal: the FP32 distances of the entries of the query's own block of 2^j leaves
-// (the smallest block holding 4k entries, leaves under one stem are contiguous) give a
-// k-th smallest S32, and every one of those k entries has an FP64 distance of at most
+// (the smallest block holding 4k entries, 64 at least for k >= 8; leaves under one stem
+// are contiguous) give a k-th smallest S32, and every one of those k entries has an FP64
+// distance of at most
 // (sqrt(S32 (1+4u)) + sqrt(K) eps)^2 (1 + 2e-15): the FP32 sum of non-negative squares is
 // at least ||delta||^2 (1-u)^K, each FP32 coordinate difference is within eps of the
 // exact one, and the FP64 evaluation is within (1 + 1e-15) of the exact square. Returns
@@ -53,13 +54,16 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
     dim = (dim + 1 == K) ? 0 : dim + 1;
   }
   const uint32_t leaf = idx - (1u << D);
+  // 4k entries, and at least 64 from k = 8 up (measured: a 64-entry block is best at k = 8
+  // and 16; for k <= 4 the smaller block is)
+  const uint32_t want = k >= 8 && 4 * k < 64 ? 64u : uint32_t(4 * k);
   uint32_t b = 0, e = 0;
   int up = 0;
   for (;; ++up) {
     const uint32_t first = (leaf >> up) << up;
     b = t.leaf_start[first];
     e = t.leaf_start[first + (1u << up)];
-    if (e - b >= uint32_t(4 * k) || up >= D) break;
+    if (e - b >= want || up >= D) break;
   }
   if (e - b < uint32_t(k)) return __builtin_huge_val();
   float top[MAXK];
