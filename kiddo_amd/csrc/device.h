@@ -242,21 +242,17 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
   for (;;) {
     // descend to a leaf (orchestrator/mod.rs:520-603)
     c.stems += uint32_t(lim - level);
+    const O bnd = vis.bound();  // the bound only changes in a leaf visit
     while (level < lim) {
+      // branch-free step: a sentinel stem (no pivot) sends the query left and queues nothing
       const A pivot = stem_at(t, idx);
-      if (pivot < Ax<A>::maxv()) {
-        const bool right = q[dim] >= pivot;
-        if (!check) {
-          pending |= 1u << level;
-        } else {
-          const O no = abs_diff(qw[dim], O(pivot));
-          const O rdf = met.rect(rd, off, dim, no);
-          if (!(rdf > vis.bound())) pending |= 1u << level;
-        }
-        idx = (idx << 1) | (right ? 1u : 0u);
-      } else {
-        idx <<= 1;  // sentinel stem: only the left child holds entries
-      }
+      const bool valid = pivot < Ax<A>::maxv();
+      const bool right = valid && q[dim] >= pivot;
+      const O no = abs_diff(qw[dim], O(pivot));
+      const O rdf = met.rect(rd, off, dim, no);
+      const bool push = valid && (!check || !(rdf > bnd));
+      pending |= (push ? 1u : 0u) << level;
+      idx = (idx << 1) | (right ? 1u : 0u);
       ++level;
       dim = (dim + 1 == K) ? 0 : dim + 1;
     }
