@@ -211,8 +211,9 @@ struct Counters {
 };
 
 // ------------------------------------------------------------------ traversal --------
-// V supplies bound(), prune(rd), unbounded() and leaf(tree, qw, first_leaf, n_leaves), which
-// scans the entries of n_leaves consecutive leaves (one, except for the first visit below).
+// V supplies bound(), prune_at(rd, bound()) (= prune(rd)), unbounded() and leaf(tree, qw,
+// first_leaf, n_leaves), which scans the entries of n_leaves consecutive leaves (one,
+// except for the first visit below).
 // first_up > 0: the first descent stops 2^first_up leaves above the bottom and that whole
 // block of leaves (the query's own) is scanned as one visit; its subtree is not entered
 // again. Any visit order finds the same canonical result (the bound only prunes what cannot
@@ -268,6 +269,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
     }
     // pop frames from the deepest level upwards (orchestrator/mod.rs:450-515)
     bool entered = false;
+    const O pbnd = vis.bound();  // constant until the next leaf visit
     while (pending | taken) {
       const int l = 31 - __clz(pending | taken);
       const uint32_t bit = 1u << l;
@@ -295,7 +297,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
       c.stems += 1;
       const O no = abs_diff(qw[ld], O(pivot));
       const O frd = met.rect(rd, off, ld, no);
-      if (vis.prune(frd)) continue;
+      if (vis.prune_at(frd, pbnd)) continue;
       taken |= bit;
       off.set(ld, no);
       rd = frd;
@@ -320,6 +322,7 @@ template <class O> struct NearestOneVis {
   uint32_t item;
   __device__ O bound() const { return best; }
   __device__ bool prune(O rd) const { return rd > best; }
+  __device__ bool prune_at(O rd, O b) const { return rd > b; }
   __device__ bool unbounded() const { return true; }
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
@@ -420,6 +423,7 @@ template <class O, int MAXK> struct KnnVis {
     return cap < b ? cap : b;
   }
   __device__ bool prune(O rd) const { return rd > bound() || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>() && cap == oinf<O>(); }
   __device__ __forceinline__ void offer(O d, uint32_t it) {
     top.insert(d, it);
@@ -512,6 +516,7 @@ template <class O> struct WithinVis {
   uint64_t cap;
   __device__ O bound() const { return radius; }
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
@@ -552,6 +557,7 @@ template <class O, int MAXK> struct BestNVis {
   }
   __device__ O bound() const { return radius; }
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
@@ -660,6 +666,7 @@ template <int MAXK> struct ScreenKnnVis {
   uint32_t rescored;
   __device__ double bound() const { return v.bound(); }
   __device__ bool prune(double rd) const { return v.prune(rd); }
+  __device__ bool prune_at(double rd, double b) const { return v.prune_at(rd, b); }
   __device__ bool unbounded() const { return v.unbounded(); }
   template <int K>
   __device__ __forceinline__ float screen_dist(const DevTree<double, K>& t, uint32_t p) const {
@@ -929,6 +936,7 @@ template <class O, int G> struct GroupWithinVis {
   uint64_t cap;
   __device__ O bound() const { return radius; }
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
@@ -986,6 +994,7 @@ template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {
   }
   __device__ O bound() const { return (cnt >= k && kd < radius) ? kd : radius; }
   __device__ bool prune(O rd) const { return rd > bound() || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
   static __device__ __forceinline__ bool less(O ad, uint32_t ai, O bd, uint32_t bi) {
     return ad < bd || (ad == bd && ai < bi);
