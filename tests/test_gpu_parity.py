@@ -310,3 +310,38 @@ def test_within_large_batch(ka, oracle_mod, cuda, r2, dense):
             assert_same_bits(r.offsets.cpu().numpy().astype(np.uint64), oo, "offsets")
             assert_same_bits(r.items.cpu().numpy(), oi, "items")
             assert_same_bits(r.distances.cpu().numpy(), od, "dists")
+
+
+SCREEN_CASES = [
+    ("grid-dups-k3", lambda rng: rng.integers(0, 7, (40_000, 3)).astype(np.float64), 32),
+    ("grid-dups-b8", lambda rng: rng.integers(0, 5, (6_000, 3)).astype(np.float64), 8),
+    ("tiny-20", lambda rng: rng.random((20, 3)), 32),
+    ("tiny-70", lambda rng: rng.random((70, 3)), 32),
+    ("uniform-k2", lambda rng: rng.random((50_000, 2)), 32),
+    ("clusters", lambda rng: np.concatenate([rng.normal(c, 0.01, (5000, 3)) for c in rng.random((8, 3))]), 16),
+]
+
+
+@pytest.mark.parametrize("name,make,B", SCREEN_CASES, ids=[c[0] for c in SCREEN_CASES])
+def test_screened_edge_cases(ka, oracle_mod, cuda, name, make, B):
+    """The FP32-screened kernel (pre-scan block as the first visit, survivor masks, entries
+    past the masks screened one at a time) against the oracle: long leaves of equal points,
+    trees smaller than the pre-scan block, k above n, 2-D trees, clusters; k from 1 to 64,
+    with and without a radius."""
+    rng = np.random.default_rng(31)
+    pts = make(rng)
+    q = _queries(pts, 3000, 5)
+    t = ka.ImmutableKdTree.new_from_slice(pts, B, shadow_f32=True)
+    O = OracleEngine(oracle_mod)
+    ot = O.build(pts, B)
+    r2 = float(np.median(O.nearest_n(ot, q[:200], 8)[1][:, -1]))
+    for k in (1, 5, 16, 32, 64):
+        for radius, excl in ((math.inf, False), (r2, False), (r2, True)):
+            _announce("screened edge", name, k, radius, excl)
+            ref = O.nearest_n(ot, q, k, radius, excl)
+            for flags in (ka.KD_SCREEN_F32, ka.KD_SCREEN_F32 | ka.KD_MORTON):
+                f = flags | (ka.KD_EXCLUSIVE if excl else 0)
+                r = t.nearest_n_raw(q, k, radius, f)
+                got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
+                for a, b, nm in zip(got, ref, ("items", "dists", "counts")):
+                    assert_same_bits(a, b, f"{name} k={k} r={radius} excl={excl} flags={f} {nm}")
