@@ -518,18 +518,28 @@ template <class O> struct WithinVis {
   __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
   __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
+  // The distances of 4 entries are computed in one branch-free block (indices clamped to
+  // the leaf), so their coordinate loads are in flight together; results follow in entry
+  // order.
   template <class A, int K, class QW>
   __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    constexpr uint32_t kChunk = 4;
     const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
-    for (uint32_t p = s; p < e; ++p) {
-      const O d = point_dist(t, p, qw);
-      const bool ok = excl ? d < radius : d <= radius;
-      if (!ok) continue;
-      if (out_items && count < cap) {
-        out_items[count] = t.items[p];
-        out_d[count] = d;
+    for (uint32_t p0 = s; p0 < e; p0 += kChunk) {
+      O dd[kChunk];
+#pragma unroll
+      for (uint32_t u = 0; u < kChunk; ++u) dd[u] = point_dist(t, p0 + u < e ? p0 + u : e - 1, qw);
+#pragma unroll
+      for (uint32_t u = 0; u < kChunk; ++u) {
+        const uint32_t p = p0 + u;
+        const bool ok = p < e && (excl ? dd[u] < radius : dd[u] <= radius);
+        if (!ok) continue;
+        if (out_items && count < cap) {
+          out_items[count] = t.items[p];
+          out_d[count] = dd[u];
+        }
+        ++count;
       }
-      ++count;
     }
   }
 };
