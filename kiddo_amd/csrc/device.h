@@ -77,6 +77,7 @@ template <class T, int K> struct Small {
 };
 
 constexpr int kMaxLevels = 32;
+constexpr int kBlock = 256;  // threads per block of every query kernel
 
 // ------------------------------------------------------------------- metrics ---------
 // Distance metrics (kiddo src/dist/*/mod.rs). Each gives dist1 of a coordinate difference
@@ -217,7 +218,8 @@ struct Counters {
 // first_up > 0: the first descent stops 2^first_up leaves above the bottom and that whole
 // block of leaves (the query's own) is scanned as one visit; its subtree is not entered
 // again. Any visit order finds the same canonical result (the bound only prunes what cannot
-// reach it); Kiddo's own order is first_up = 0.
+// reach it); Kiddo's own order is first_up = 0. When the DFS is done, the visitor gets one
+// more visit of no leaves (n_leaves = 0), in which it may finish deferred work.
 template <class A, int K, class V, class QW>
 __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, K>& q, const QW& qw, V& vis,
                                          Counters& c, int first_up = 0) {
@@ -240,6 +242,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
   int level = 0, dim = 0;
   bool check = !vis.unbounded();
   int lim = D - (first_up < D ? first_up : D);
+  bool final = false;
   for (;;) {
     // descend to a leaf (orchestrator/mod.rs:520-603)
     c.stems += uint32_t(lim - level);
@@ -258,15 +261,16 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
       dim = (dim + 1 == K) ? 0 : dim + 1;
     }
     {
-      const uint32_t nl = 1u << (D - lim);
+      const uint32_t nl = final ? 0u : 1u << (D - lim);
       idx <<= (D - lim);  // the block's first leaf: its ancestors above the block are unchanged
       level = D;
       lim = D;
       const uint32_t leaf = idx - (1u << D);
       c.leaves += nl;
       c.points += t.leaf_start[leaf + nl] - t.leaf_start[leaf];
-      vis.leaf(t, qw, leaf, nl);
+      vis.leaf(t, qw, leaf, nl);  // the one call site: a second one doubles the top-k's registers
     }
+    if (final) return;
     // pop frames from the deepest level upwards (orchestrator/mod.rs:450-515)
     bool entered = false;
     const O pbnd = vis.bound();  // constant until the next leaf visit
@@ -308,7 +312,7 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
       entered = true;
       break;
     }
-    if (!entered) return;
+    if (!entered) final = true;
   }
 }
 
@@ -648,6 +652,12 @@ template <int W> struct EntryMask {
     for (int w = 0; w < W; ++w)
       if (j0 >= 64u * uint32_t(w) && j0 < 64u * uint32_t(w + 1)) m[w] |= uint64_t(bits) << (j0 - 64u * uint32_t(w));
   }
+  __device__ __forceinline__ uint32_t count() const {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c += uint32_t(__popcll(m[w]));
+    return c;
+  }
   __device__ __forceinline__ bool any() const {
     uint64_t a = 0ull;
 #pragma unroll
@@ -674,6 +684,7 @@ template <int MAXK> struct ScreenKnnVis {
   double eps_k;
   float thr;
   uint32_t rescored;
+  uint32_t visits, qhead, qn;  // visits so far; deferred survivors (a ring in LDS)
   __device__ double bound() const { return v.bound(); }
   __device__ bool prune(double rd) const { return v.prune(rd); }
   __device__ bool prune_at(double rd, double b) const { return v.prune_at(rd, b); }
@@ -700,18 +711,30 @@ template <int MAXK> struct ScreenKnnVis {
     while (next < L && screen_dist(t, s + next) > thr) ++next;
     return next;
   }
+  // Survivors past kDeferBudget in a visit after the first are deferred: their entry
+  // indices wait in an LDS ring of kDeferQ per lane and are offered in later visits within
+  // the same budget, or in the final visit (n_leaves = 0), which drains them. A wave steps
+  // through a visit's survivors at the pace of its busiest lane, so the budget bounds the
+  // steps per visit. Deferring only delays tightening the bound (more is visited, nothing
+  // is lost): a survivor is measured in FP64 and checked against the bound of its turn, and
+  // an entry of the final result passes every bound.
+  // measured (16.8M points, k = 16 / 8 / 4): budgets 2 / 4 / 6 / 9 give 24.3 / 20.1 / 20.9 /
+  // 21.7 ms at k = 16; small top-k lists gain less from deferring
+  static constexpr uint32_t kDeferBudget = MAXK >= 8 ? 4u : 8u, kDeferQ = 16;
   // Two passes per visit. The FP32 screen marks the survivors among the first 128 entries (64
   // below a top-k of 16, whose first visit, the pre-scan block, is smaller) in a bit mask
   // against the threshold at the start of the visit (shadow loads issued 4 at a time,
   // indices clamped to the visit); then only survivors are measured in FP64 and offered, in
-  // entry order, to the collector, one per step, the next one's loads issued a step ahead.
-  // Entries past the mask are screened one at a time when the mask is empty. The top-k is
-  // updated at one place inside one loop: a second update site, or a loop around this one,
+  // entry order, to the collector, one per step. Entries past the mask are screened one at
+  // a time when the mask is empty, deferred ones come last. The top-k is updated at one
+  // place inside one loop whose exit test is at the top (the next survivor is chosen at the
+  // end of a step): a second update site, a loop around this one or an exit in the middle
   // makes the compiler hold a second copy of the top-k in registers.
   template <int K, class QW>
   __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    __shared__ uint32_t deferred[kDeferQ * kBlock];
     constexpr uint32_t kBatch = 4;
-    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
+    const uint32_t s = t.leaf_start[leaf], e = nl ? t.leaf_start[leaf + nl] : s;
     constexpr int kWords = MAXK >= 16 ? 2 : 1;
     constexpr uint32_t kMaskN = 64u * kWords;
     const uint32_t L = e - s, nm = L < kMaskN ? L : kMaskN;
@@ -739,32 +762,47 @@ template <int MAXK> struct ScreenKnnVis {
       }
       mk.set4(j0, bits);  // j0 is a multiple of 4: the 4 bits never straddle two words
     }
+    const uint32_t lane = threadIdx.x;
+    const uint32_t budget = (visits == 0u || nl == 0u) ? 0xFFFFFFFFu : kDeferBudget;
+    visits += 1u;
+    for (uint32_t c = mk.count(); c > budget && qn < kDeferQ; --c) {
+      deferred[((qhead + qn) % kDeferQ) * kBlock + lane] = s + mk.pop();
+      ++qn;
+    }
     uint32_t next = nm;  // first entry not covered by the mask
-    // one survivor ahead: its loads are issued before the current one is offered (every
-    // step is branch-free apart from the tail screen, so they can overlap the insertion)
-    bool inmask = mk.any();
-    if (!inmask) next = next_tail(t, s, next, L);
-    bool have = inmask || next < L;
-    uint32_t j = inmask ? mk.pop() : next;
-    next += (!inmask && have) ? 1u : 0u;
-    double dn = have ? point_dist(t, s + j, qw) : 0.0;
-    uint32_t in = have ? t.items[s + j] : 0u;
+    uint32_t used = 0, p = 0;
+    bool have = true;
+    // the next survivor: the mask, then the entries past it, then deferred ones
+    auto select = [&]() {
+      have = true;
+      if (mk.any()) {
+        p = s + mk.pop();
+      } else {
+        next = next_tail(t, s, next, L);
+        if (next < L) {
+          p = s + next;
+          ++next;
+        } else if (qn > 0u && used < budget) {
+          p = deferred[qhead * kBlock + lane];
+          qhead = (qhead + 1u) % kDeferQ;
+          --qn;
+        } else {
+          have = false;
+        }
+      }
+    };
+    select();
     while (have) {
-      const double d = dn;
-      const uint32_t it = in;
-      const uint32_t jcur = j;
-      inmask = mk.any();
-      if (!inmask) next = next_tail(t, s, next, L);
-      have = inmask || next < L;
-      j = inmask ? mk.pop() : (next < L ? next : L - 1);
-      next += (!inmask && have) ? 1u : 0u;
-      dn = point_dist(t, s + j, qw);  // in bounds: j < L (a repeat of the last entry when done)
-      in = t.items[s + j];
+      const uint32_t pc = p;
+      ++used;
+      select();
       ++rescored;
+      const uint32_t it = t.items[pc];
+      const double d = point_dist(t, pc, qw);
       bool ok = (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
       ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
       v.offer_if(ok, d, it);
-      if (jcur + 1u >= kMaskN) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
+      if (pc - s + 1u >= kMaskN) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
     }
     // The bound only tightens, so a threshold from the start of the visit stays valid for
     // the mask; refresh it once for the next visit.
@@ -801,7 +839,6 @@ __device__ __forceinline__ void load_query(const A* __restrict__ queries, uint64
   }
 }
 
-constexpr int kBlock = 256;
 
 // Kernel prologue shared by every family: one thread per query, queries optionally
 // visited in a permuted (Morton) order; results always land at the query's own index.

@@ -120,6 +120,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     v.v.cap = prescan_cap<K, MAXK>(t, q, v.qf, v.eps_k, k, up);
     v.thr = screen_threshold(v.v.bound(), v.eps_k);
     v.rescored = 0;
+    v.visits = 0;
+    v.qhead = 0;
+    v.qn = 0;
     traverse(t, q, qw, v, cnt, up);
     uint32_t* ri = out_items + qi * uint64_t(k);
     double* rdd = out_d + qi * uint64_t(k);
