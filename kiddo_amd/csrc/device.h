@@ -792,13 +792,19 @@ template <int MAXK> struct ScreenKnnVis {
       }
     };
     select();
+    double dn = have ? point_dist(t, p, qw) : 0.0;
+    uint32_t in = have ? t.items[p] : 0u;
     while (have) {
       const uint32_t pc = p;
+      const double d = dn;
+      const uint32_t it = in;
       ++used;
       select();
+      // the next survivor's loads are issued before this one is offered
+      const uint32_t pn = have ? p : pc;
+      dn = point_dist(t, pn, qw);
+      in = t.items[pn];
       ++rescored;
-      const uint32_t it = t.items[pc];
-      const double d = point_dist(t, pc, qw);
       bool ok = (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
       ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
       v.offer_if(ok, d, it);
