@@ -278,7 +278,10 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
       const int l = 31 - __clz(pending | taken);
       const uint32_t bit = 1u << l;
       if (taken & bit) {
-        taken ^= bit;
+        // every taken level below the deepest pending one is popped before it: clear them
+        // together and rebuild once
+        const uint32_t upto = pending ? (2u << (31 - __clz(pending))) - 1u : 0u;
+        taken &= upto;
         // rebuild rd and off[] from the remaining taken levels, top-down
         rd = O(0);
         off.fill(O(0));
