@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_forest_within(
 
 __global__ void k_forest_reset(unsigned int* queue, unsigned long long* stats) {
   if (threadIdx.x == 0) *queue = 0u;
-  if (threadIdx.x < 4) stats[threadIdx.x] = 0ull;
+  for (uint32_t i = threadIdx.x; i < 4u * kStatSlots; i += blockDim.x) stats[i] = 0ull;
 }
 
 // Morton keys tagged with the query's tree: sorting (tree, key) pairs reorders queries only
@@ -446,7 +446,7 @@ int forest_create_typed(const kd_tree_desc* descs, uint32_t ntrees, kd_forest** 
   alloc_up((void**)&f->items, items.data(), items.size() * 4);
   alloc_up((void**)&f->trees, f->h_trees.data(), f->h_trees.size() * sizeof(dev::ForestTree));
   alloc_up((void**)&f->bbox, f->h_bbox.data(), f->h_bbox.size() * sizeof(double));
-  alloc_up((void**)&f->counters, nullptr, 4 * sizeof(unsigned long long));
+  alloc_up((void**)&f->counters, nullptr, 4 * kd::dev::kStatSlots * sizeof(unsigned long long));
   alloc_up((void**)&f->queue, nullptr, 64);
   if (rc) {
     free_forest(f);
@@ -823,10 +823,11 @@ int kd_forest_last_kernel_ms(kd_forest* f, float* ms) {
 
 int kd_forest_query_stats(kd_forest* f, uint64_t* out4) {
   if (!f || !out4) return kd::set_error(KD_ERR_INVALID, "null argument");
-  unsigned long long h[4] = {0, 0, 0, 0};
+  std::vector<unsigned long long> h(4 * kd::dev::kStatSlots);
   KD_HIP(hipDeviceSynchronize());
-  KD_HIP(hipMemcpy(h, f->counters, sizeof(h), hipMemcpyDeviceToHost));
-  for (int j = 0; j < 4; ++j) out4[j] = h[j];
+  KD_HIP(hipMemcpy(h.data(), f->counters, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost));
+  for (int j = 0; j < 4; ++j) out4[j] = 0;
+  for (size_t i = 0; i < h.size(); ++i) out4[i % 4] += h[i];
   return KD_OK;
 }
 
