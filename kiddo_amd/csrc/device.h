@@ -722,8 +722,8 @@ template <int MAXK> struct ScreenKnnVis {
   // steps per visit. Deferring only delays tightening the bound (more is visited, nothing
   // is lost): a survivor is measured in FP64 and checked against the bound of its turn, and
   // an entry of the final result passes every bound.
-  // measured (16.8M points, k = 16 / 8 / 4): budgets 2 / 4 / 6 / 9 give 24.3 / 20.1 / 20.9 /
-  // 21.7 ms at k = 16; small top-k lists gain less from deferring
+  // Measured on 16.8M points, k = 16: budgets 2 / 4 / 6 / 9 took 24.3 / 20.1 / 20.9 / 21.7 ms;
+  // small top-k lists gain less from deferring.
   static constexpr uint32_t kDeferBudget = MAXK >= 8 ? 4u : 8u, kDeferQ = 16;
   // Two passes per visit. The FP32 screen marks the survivors among the first 128 entries (64
   // below a top-k of 16, whose first visit, the pre-scan block, is smaller) in a bit mask
@@ -812,7 +812,9 @@ template <int MAXK> struct ScreenKnnVis {
       bool ok = (v.excl ? d < v.radius : d <= v.radius) && d <= v.cap;
       ok = ok && (v.cnt < v.k || TopK<double, MAXK>::less(d, it, v.kd, v.kit));
       v.offer_if(ok, d, it);
-      if (pc - s + 1u >= kMaskN) thr = screen_threshold(v.bound(), eps_k);  // the tail screens with it
+      // entries past the mask are screened with the current bound (for a deferred entry the
+      // difference wraps and the refresh is merely early)
+      if (pc - s + 1u >= kMaskN) thr = screen_threshold(v.bound(), eps_k);
     }
     // The bound only tightens, so a threshold from the start of the visit stays valid for
     // the mask; refresh it once for the next visit.
