@@ -58,10 +58,9 @@ __device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(
 __device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
 __device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
 
-// |a - b| in O (Axis::saturating_dist for floats, axis.rs:124; exact for widened ints)
-template <class O> __device__ __forceinline__ O abs_diff(O a, O b) {
-  return a >= b ? sub_rn(a, b) : sub_rn(b, a);
-}
+// |a - b| in O: Axis::saturating_dist for floats, (a - b).abs() (axis.rs:124-126); for
+// integer axes (larger minus smaller, axis.rs:191-197) the widened difference is exact.
+template <class O> __device__ __forceinline__ O abs_diff(O a, O b) { return fabs(sub_rn(a, b)); }
 
 // A K-vector (K <= 4) indexed by a run-time dimension. It is held as one vector value, so
 // an element access is an extract/insert on registers that the backend expands into
@@ -244,13 +243,14 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
   bool check = !vis.unbounded();
   int lim = D - (first_up < D ? first_up : D);
   bool final = false;
+  // levels whose stems are all in LDS: level l holds indices [2^l, 2^(l+1))
+  const int lds_levels = t.lds_n ? 31 - __clz(t.lds_n) : 0;
   for (;;) {
     // descend to a leaf (orchestrator/mod.rs:520-603)
     c.stems += uint32_t(lim - level);
     const O bnd = vis.bound();  // the bound only changes in a leaf visit
-    while (level < lim) {
-      // branch-free step: a sentinel stem (no pivot) sends the query left and queues nothing
-      const A pivot = stem_at(t, idx);
+    // branch-free step: a sentinel stem (no pivot) sends the query left and queues nothing
+    auto step = [&](const A pivot) {
       const bool valid = pivot < Ax<A>::maxv();
       const bool right = valid && q[dim] >= pivot;
       const O no = abs_diff(qw[dim], O(pivot));
@@ -260,7 +260,12 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
       idx = (idx << 1) | (right ? 1u : 0u);
       ++level;
       dim = (dim + 1 == K) ? 0 : dim + 1;
-    }
+    };
+    // the levels held in LDS, then the rest: each loop reads one kind of memory (a load
+    // that picks LDS or global memory per step needs a generic address built every step)
+    const int mid = lim < lds_levels ? lim : lds_levels;
+    while (level < mid) step(t.lds[idx]);
+    while (level < lim) step(t.stems[idx]);
     {
       const uint32_t nl = final ? 0u : 1u << (D - lim);
       idx <<= (D - lim);  // the block's first leaf: its ancestors above the block are unchanged
@@ -286,16 +291,22 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
         // rebuild rd and off[] from the remaining taken levels, top-down
         rd = O(0);
         off.fill(O(0));
-        uint32_t rest = taken;
-        while (rest) {
-          const int tl = __ffs(rest) - 1;
-          rest &= rest - 1u;
+        auto replay = [&](const int tl, const A pv) {
           const int td = tl % K;
-          const A pv = stem_at(t, idx >> (D - tl));
           c.stems += 1;
           const O no = abs_diff(qw[td], O(pv));
           rd = met.rect(rd, off, td, no);
           off.set(td, no);
+        };
+        // the LDS levels, then the rest (one kind of memory per loop, as in the descent)
+        const uint32_t lds_mask = (1u << lds_levels) - 1u;
+        for (uint32_t rest = taken & lds_mask; rest; rest &= rest - 1u) {
+          const int tl = __ffs(rest) - 1;
+          replay(tl, t.lds[idx >> (D - tl)]);
+        }
+        for (uint32_t rest = taken & ~lds_mask; rest; rest &= rest - 1u) {
+          const int tl = __ffs(rest) - 1;
+          replay(tl, t.stems[idx >> (D - tl)]);
         }
         continue;
       }
@@ -385,6 +396,40 @@ template <class O, int MAXK> struct TopK {
     it[0] = lt[0] ? ci : it[0];
     d[0] = d[0] < c ? d[0] : c;
   }
+  // Batcher's odd-even merge sort over the slots in (distance, item) order: the same list
+  // as inserting the entries one at a time into an empty list (both keep the MAXK smallest
+  // in order), for (MAXK/4) (log2 MAXK)^2 compare-exchanges instead of MAXK insertions of
+  // MAXK slots each.
+  __device__ __forceinline__ void sort() { sort_step<0>(); }
+  template <int C> __device__ __forceinline__ void sort_step() {
+    if constexpr (C < kNet.n) {
+      constexpr int a = kNet.a[C], b = kNet.b[C];
+      const bool sw = less(d[b], it[b], d[a], it[a]);
+      const O lo = fmin(d[a], d[b]), hi = fmax(d[a], d[b]);  // equal distances: either
+      const uint32_t il = sw ? it[b] : it[a], ih = sw ? it[a] : it[b];
+      d[a] = lo;
+      d[b] = hi;
+      it[a] = il;
+      it[b] = ih;
+      sort_step<C + 1>();
+    }
+  }
+  struct Net {
+    int n = 0;
+    unsigned char a[MAXK * 16] = {}, b[MAXK * 16] = {};
+    constexpr Net() {
+      for (int p = 1; p < MAXK; p <<= 1)
+        for (int k = p; k >= 1; k >>= 1)
+          for (int j = k % p; j + k < MAXK; j += 2 * k)
+            for (int i = 0; i < k && i + j + k < MAXK; ++i)
+              if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                a[n] = (unsigned char)(i + j);
+                b[n] = (unsigned char)(i + j + k);
+                ++n;
+              }
+    }
+  };
+  static constexpr Net kNet{};
   __device__ __forceinline__ void kth(int k, O& kd, uint32_t& kit) const {
     if (k == MAXK) {
       kd = d[MAXK - 1];
@@ -689,6 +734,7 @@ template <int MAXK> struct ScreenKnnVis {
   float thr;
   uint32_t rescored;
   uint32_t visits, qhead, qn;  // visits so far; deferred survivors (a ring in LDS)
+  uint32_t skip;                // entries of the first visit already offered (first_block)
   __device__ double bound() const { return v.bound(); }
   __device__ bool prune(double rd) const { return v.prune(rd); }
   __device__ bool prune_at(double rd, double b) const { return v.prune_at(rd, b); }
@@ -725,29 +771,15 @@ template <int MAXK> struct ScreenKnnVis {
   // Measured on 16.8M points, k = 16: budgets 2 / 4 / 6 / 9 took 24.3 / 20.1 / 20.9 / 21.7 ms;
   // small top-k lists gain less from deferring.
   static constexpr uint32_t kDeferBudget = MAXK >= 8 ? 4u : 8u, kDeferQ = 16;
-  // Two passes per visit. The FP32 screen marks the survivors among the first 128 entries (64
-  // below a top-k of 16, whose first visit, the pre-scan block, is smaller) in a bit mask
-  // against the threshold at the start of the visit (shadow loads issued 4 at a time,
-  // indices clamped to the visit); then only survivors are measured in FP64 and offered, in
-  // entry order, to the collector, one per step. Entries past the mask are screened one at
-  // a time when the mask is empty, deferred ones come last. The top-k is updated at one
-  // place inside one loop whose exit test is at the top (the next survivor is chosen at the
-  // end of a step): a second update site, a loop around this one or an exit in the middle
-  // makes the compiler hold a second copy of the top-k in registers.
-  template <int K, class QW>
-  __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
-    __shared__ uint32_t deferred[kDeferQ * kBlock];
+  static constexpr int kWords = MAXK >= 16 ? 2 : 1;
+  static constexpr uint32_t kMaskN = 64u * kWords;
+  // marks in mk the entries among [s, s + nm) whose FP32 distance passes thr (shadow loads
+  // issued 4 at a time, indices clamped to the range)
+  template <int K>
+  __device__ __forceinline__ void screen_into(const DevTree<double, K>& t, uint32_t s, uint32_t nm,
+                                              EntryMask<kWords>& mk) const {
     constexpr uint32_t kBatch = 4;
-    const uint32_t s = t.leaf_start[leaf], e = nl ? t.leaf_start[leaf + nl] : s;
-    constexpr int kWords = MAXK >= 16 ? 2 : 1;
-    constexpr uint32_t kMaskN = 64u * kWords;
-    const uint32_t L = e - s, nm = L < kMaskN ? L : kMaskN;
-    // unbounded (top-k not yet full, no radius): every entry survives the screen
-    const bool screen = thr < __builtin_huge_valf();
-    EntryMask<kWords> mk;
-    mk.clear();
-    if (!screen) mk.set_first(nm);
-    for (uint32_t j0 = 0; screen && j0 < nm; j0 += kBatch) {
+    for (uint32_t j0 = 0; j0 < nm; j0 += kBatch) {
       float4 sh[kBatch];
 #pragma unroll
       for (uint32_t u = 0; u < kBatch; ++u) sh[u] = t.shadow[s + (j0 + u < nm ? j0 + u : nm - 1)];
@@ -766,6 +798,76 @@ template <int MAXK> struct ScreenKnnVis {
       }
       mk.set4(j0, bits);  // j0 is a multiple of 4: the 4 bits never straddle two words
     }
+  }
+  // The first visit's block [b, e) (the pre-scan's, whose cap is the bound; b = e = 0 when
+  // there is none; the tree must not be empty) offered before the traversal: the survivors of the screen among its first
+  // kMaskN entries, up to MAXK of them, are measured in FP64 together and sorted into the
+  // empty top-k, which is the list that offering them one at a time builds. `skip` records
+  // how far into the block this reached (up to the last survivor taken when more remain);
+  // the traversal's visit of the block starts there.
+  template <int K, class QW>
+  __device__ __forceinline__ void first_block(const DevTree<double, K>& t, const QW& qw, uint32_t b, uint32_t e) {
+    const uint32_t L = e - b, nm = L < kMaskN ? L : kMaskN;
+    EntryMask<kWords> mk;
+    mk.clear();
+    screen_into(t, b, nm, mk);
+    // the survivors' indices first (an empty slot reads entry b, or entry 0 of a non-empty
+    // tree when the block is empty), so that all their loads can be in flight together
+    uint32_t pidx[MAXK];
+    uint32_t last = 0, taken = 0;
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) {
+      const bool valid = mk.any();
+      const uint32_t r = mk.pop();
+      last = valid ? r : last;
+      taken += valid ? 1u : 0u;
+      pidx[j] = b + (valid ? r : 0u);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) {
+      const double dd = point_dist(t, pidx[j], qw);
+      const uint32_t it = t.items[pidx[j]];
+      // bitwise, not short-circuit: a branch here lets the loads sink under it, one at a time
+      const bool ok = (uint32_t(j) < taken) & ((dd < v.radius) | (!v.excl & (dd == v.radius))) & (dd <= v.cap);
+      // The item is masked rather than selected: the compiler turns a select of a loaded
+      // value into a branch around its load, one candidate at a time. The empty asm keeps
+      // the mask opaque so that it cannot fold the OR back into such a select.
+      uint32_t drop = ok ? 0u : 0xFFFFFFFFu;
+      asm("" : "+v"(drop));
+      v.top.d[j] = ok ? dd : oinf<double>();
+      v.top.it[j] = it | drop;
+    }
+    v.top.sort();
+    int nok = 0;
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) nok += v.top.d[j] < oinf<double>() ? 1 : 0;
+    v.cnt = nok < v.k ? nok : v.k;
+    if (v.cnt >= v.k) v.top.kth(v.k, v.kd, v.kit);
+    rescored += taken;
+    skip = mk.any() ? last + 1u : nm;
+    thr = screen_threshold(v.bound(), eps_k);
+  }
+  // Two passes per visit. The FP32 screen marks the survivors among the first 128 entries (64
+  // below a top-k of 16, whose first visit, the pre-scan block, is smaller) in a bit mask
+  // against the threshold at the start of the visit (shadow loads issued 4 at a time,
+  // indices clamped to the visit); then only survivors are measured in FP64 and offered, in
+  // entry order, to the collector, one per step. Entries past the mask are screened one at
+  // a time when the mask is empty, deferred ones come last. The top-k is updated at one
+  // place inside one loop whose exit test is at the top (the next survivor is chosen at the
+  // end of a step): a second update site, a loop around this one or an exit in the middle
+  // makes the compiler hold a second copy of the top-k in registers.
+  template <int K, class QW>
+  __device__ void leaf(const DevTree<double, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    __shared__ uint32_t deferred[kDeferQ * kBlock];
+    const uint32_t s0 = t.leaf_start[leaf], e = nl ? t.leaf_start[leaf + nl] : s0;
+    const uint32_t s = visits == 0u ? s0 + skip : s0;
+    const uint32_t L = e - s, nm = L < kMaskN ? L : kMaskN;
+    // unbounded (top-k not yet full, no radius): every entry survives the screen
+    const bool screen = thr < __builtin_huge_valf();
+    EntryMask<kWords> mk;
+    mk.clear();
+    if (!screen) mk.set_first(nm);
+    screen_into(t, s, screen ? nm : 0u, mk);
     const uint32_t lane = threadIdx.x;
     const uint32_t budget = (visits == 0u || nl == 0u) ? 0xFFFFFFFFu : kDeferBudget;
     visits += 1u;

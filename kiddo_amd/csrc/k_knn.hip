@@ -39,20 +39,26 @@ __device__ __forceinline__ void screen_setup(const Small<double, K>& q, const do
 // at least ||delta||^2 (1-u)^K, each FP32 coordinate difference is within eps of the
 // exact one, and the FP64 evaluation is within (1 + 1e-15) of the exact square. Returns
 // +inf when the tree holds fewer than k entries. `up` receives the block's height in
-// levels (2^up leaves), 0 with an infinite cap.
+// levels (2^up leaves) and [b, e) its entries; 0 and an empty range with an infinite cap.
 template <int K, int MAXK>
 __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K>& q, const float (&qf)[3],
-                              double eps_k, int k, int& up_out) {
+                              double eps_k, int k, int& up_out, uint32_t& b_out, uint32_t& e_out) {
   up_out = 0;
+  b_out = e_out = 0;
   if (t.n < uint32_t(k)) return __builtin_huge_val();
   const int D = t.nlev;
   uint32_t idx = 1;
-  int dim = 0;
-  for (int level = 0; level < D; ++level) {
-    const double pivot = stem_at(t, idx);
+  int dim = 0, level = 0;
+  auto step = [&](const double pivot) {
     idx = (idx << 1) | (q[dim] >= pivot ? 1u : 0u);
     dim = (dim + 1 == K) ? 0 : dim + 1;
-  }
+    ++level;
+  };
+  // the levels held in LDS, then the rest (one kind of memory per loop)
+  const int lds_levels = t.lds_n ? 31 - __clz(t.lds_n) : 0;
+  const int mid = D < lds_levels ? D : lds_levels;
+  while (level < mid) step(t.lds[idx]);
+  while (level < D) step(t.stems[idx]);
   const uint32_t leaf = idx - (1u << D);
   // 4k entries, and at least 64 from k = 8 up (measured: a 64-entry block is best at k = 8
   // and 16; for k <= 4 the smaller block is)
@@ -98,6 +104,8 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
   for (int j = 1; j < MAXK; ++j) kth = (j == k - 1) ? top[j] : kth;
   if (!(kth < __builtin_huge_valf())) return __builtin_huge_val();
   up_out = up;
+  b_out = b;
+  e_out = e;
   const double r = sqrt(double(kth) * (1.0 + 4.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
   return r * r * (1.0 + 2e-15);
 }
@@ -115,14 +123,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     v.v.init(k, radius, excl != 0);
     screen_setup<K>(q, max_abs, v.qf, v.eps_k);
     // the pre-scan's block is then the first visit: its entries within the cap are the first
-    // candidates of every lane, and its leaves are not visited again
+    // candidates of every lane (sorted in together before the traversal), and its leaves are
+    // not visited again
     int up = 0;
-    v.v.cap = prescan_cap<K, MAXK>(t, q, v.qf, v.eps_k, k, up);
+    uint32_t bb = 0, be = 0;
+    v.v.cap = prescan_cap<K, MAXK>(t, q, v.qf, v.eps_k, k, up, bb, be);
     v.thr = screen_threshold(v.v.bound(), v.eps_k);
     v.rescored = 0;
     v.visits = 0;
     v.qhead = 0;
     v.qn = 0;
+    if (t.n != 0) v.first_block(t, qw, bb, be);
     traverse(t, q, qw, v, cnt, up);
     uint32_t* ri = out_items + qi * uint64_t(k);
     double* rdd = out_d + qi * uint64_t(k);
