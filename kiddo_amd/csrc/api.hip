@@ -164,7 +164,8 @@ template <class A, int K> int create_typed(const kd_tree_desc* desc, uint32_t fl
   const uint64_t plane = (n + 63) / 64 * 64;
   std::vector<A> coords(plane * K);
   std::vector<uint32_t> items(n);
-  std::vector<float4> shadow((flags & KD_TREE_SHADOW_F32) ? n : 0);
+  // zeroed padding past the last point (dev::kShadowPad)
+  std::vector<float4> shadow((flags & KD_TREE_SHADOW_F32) ? n + dev::kShadowPad : 0, float4{0.0f, 0.0f, 0.0f, 0.0f});
   // per-thread partial bounding boxes
   std::mutex mu;
   double lo[4] = {INFINITY, INFINITY, INFINITY, INFINITY}, hi[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};

@@ -78,6 +78,9 @@ template <class T, int K> struct Small {
 constexpr int kMaxLevels = 32;
 constexpr int kBlock = 256;  // threads per block of every query kernel
 constexpr int kStatSlots = 256;  // query-statistics counters: groups of 4, summed on read
+// The FP32 shadow array holds this many (zero) entries past the last point, so that batched
+// loads may run past the end of a leaf without clamping their indices.
+constexpr uint32_t kShadowPad = 8;
 
 // ------------------------------------------------------------------- metrics ---------
 // Distance metrics (kiddo src/dist/*/mod.rs). Each gives dist1 of a coordinate difference
@@ -156,7 +159,7 @@ template <class A, int K> struct DevTree {
   const uint32_t* __restrict__ leaf_start;
   const A* __restrict__ coords;  // K planes of `plane` entries
   const uint32_t* __restrict__ items;
-  const float4* __restrict__ shadow;
+  const float4* __restrict__ shadow;  // n + kShadowPad entries
   const A* lds;     // the first lds_n stems, copied into LDS by the kernel prologue
   uint32_t lds_n;   // 0: no LDS copy
   uint64_t plane;
@@ -773,30 +776,38 @@ template <int MAXK> struct ScreenKnnVis {
   static constexpr uint32_t kDeferBudget = MAXK >= 8 ? 4u : 8u, kDeferQ = 16;
   static constexpr int kWords = MAXK >= 16 ? 2 : 1;
   static constexpr uint32_t kMaskN = 64u * kWords;
-  // marks in mk the entries among [s, s + nm) whose FP32 distance passes thr (shadow loads
-  // issued 4 at a time, indices clamped to the range)
+  // marks in mk the entries among [s, s + nm) whose FP32 distance passes thr: shadow loads
+  // issued 4 at a time from one address (up to 3 past the range: the shadow's padding),
+  // bits gathered per 64-entry word
   template <int K>
   __device__ __forceinline__ void screen_into(const DevTree<double, K>& t, uint32_t s, uint32_t nm,
                                               EntryMask<kWords>& mk) const {
     constexpr uint32_t kBatch = 4;
-    for (uint32_t j0 = 0; j0 < nm; j0 += kBatch) {
-      float4 sh[kBatch];
 #pragma unroll
-      for (uint32_t u = 0; u < kBatch; ++u) sh[u] = t.shadow[s + (j0 + u < nm ? j0 + u : nm - 1)];
-      uint32_t bits = 0;
+    for (int w = 0; w < kWords; ++w) {
+      const uint32_t lo = 64u * uint32_t(w), hi = nm < lo + 64u ? nm : lo + 64u;
+      uint64_t word = 0;
+      for (uint32_t j0 = lo; j0 < hi; j0 += kBatch) {
+        const float4* __restrict__ sp = t.shadow + s + j0;
+        float4 sh[kBatch];
 #pragma unroll
-      for (uint32_t u = 0; u < kBatch; ++u) {
-        const float dx = sub_rn(sh[u].x, qf[0]);
-        float acc = mul_rn(dx, dx);
-        const float dy = sub_rn(sh[u].y, qf[1]);
-        acc = add_rn(acc, mul_rn(dy, dy));
-        if (K > 2) {
-          const float dz = sub_rn(sh[u].z, qf[2]);
-          acc = add_rn(acc, mul_rn(dz, dz));
+        for (uint32_t u = 0; u < kBatch; ++u) sh[u] = sp[u];
+        uint32_t bits = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+          const float dx = sub_rn(sh[u].x, qf[0]);
+          float acc = mul_rn(dx, dx);
+          const float dy = sub_rn(sh[u].y, qf[1]);
+          acc = add_rn(acc, mul_rn(dy, dy));
+          if (K > 2) {
+            const float dz = sub_rn(sh[u].z, qf[2]);
+            acc = add_rn(acc, mul_rn(dz, dz));
+          }
+          bits |= ((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << u;
         }
-        bits |= ((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << u;
+        word |= uint64_t(bits) << (j0 - lo);
       }
-      mk.set4(j0, bits);  // j0 is a multiple of 4: the 4 bits never straddle two words
+      mk.m[w] |= word;
     }
   }
   // The first visit's block [b, e) (the pre-scan's, whose cap is the bound; b = e = 0 when

@@ -379,7 +379,10 @@ int build_typed(kd_tree* t, const A* pts, const uint32_t* items, uint64_t n, int
   KD_B(dev_alloc(reinterpret_cast<void**>(&t->d_max_abs), 4 * sizeof(double)));
   KD_B(dev_alloc(reinterpret_cast<void**>(&t->d_bbox), 8 * sizeof(double)));
   KD_B(dev_alloc(reinterpret_cast<void**>(&t->counters), 4 * dev::kStatSlots * sizeof(unsigned long long)));
-  if (flags & KD_TREE_SHADOW_F32) KD_B(dev_alloc(reinterpret_cast<void**>(&t->shadow), std::max<uint64_t>(n, 1) * 16));
+  if (flags & KD_TREE_SHADOW_F32) {
+    KD_B(dev_alloc(reinterpret_cast<void**>(&t->shadow), (n + dev::kShadowPad) * 16));
+    KD_HIP(hipMemsetAsync(t->shadow + n, 0, dev::kShadowPad * 16, s));  // the padding
+  }
   KD_HIP(hipMemcpyAsync(t->leaf_start, leaf_start.data(), (leaves + 1) * 4, hipMemcpyHostToDevice, s));
   if (n > 0)
     hipLaunchKernelGGL((dev::k_gather_tree<A, K>), dim3(grid_for(n)), dim3(dev::kBlock), 0, s, pts, items, P, n,

@@ -80,7 +80,7 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
     float d32[kBatch];
 #pragma unroll
     for (uint32_t u = 0; u < kBatch; ++u) {
-      const float4 sh = t.shadow[p0 + u < e ? p0 + u : e - 1];
+      const float4 sh = t.shadow[p0 + u];  // up to 7 past e: the shadow's padding
       const float dx = sub_rn(sh.x, qf[0]);
       float acc = mul_rn(dx, dx);
       const float dy = sub_rn(sh.y, qf[1]);
