@@ -858,12 +858,12 @@ template <int MAXK> struct ScreenKnnVis {
     skip = mk.any() ? last + 1u : nm;
     thr = screen_threshold(v.bound(), eps_k);
   }
-  // Two passes per visit. The FP32 screen marks the survivors among the first 128 entries (64
-  // below a top-k of 16, whose first visit, the pre-scan block, is smaller) in a bit mask
-  // against the threshold at the start of the visit (shadow loads issued 4 at a time,
-  // indices clamped to the visit); then only survivors are measured in FP64 and offered, in
-  // entry order, to the collector, one per step. Entries past the mask are screened one at
-  // a time when the mask is empty, deferred ones come last. The top-k is updated at one
+  // Two passes per visit. The FP32 screen (screen_into) marks the survivors among the first
+  // kMaskN entries (128 for a top-k of 16 or more, 64 below) in a bit mask against the
+  // threshold at the start of the visit; then only survivors are measured in FP64 and
+  // offered, in entry order, to the collector, one per step. Entries past the mask are
+  // screened one at a time when the mask is empty, deferred ones come last. The first visit
+  // (the pre-scan block) starts after the entries first_block offered. The top-k is updated at one
   // place inside one loop whose exit test is at the top (the next survivor is chosen at the
   // end of a step): a second update site, a loop around this one or an exit in the middle
   // makes the compiler hold a second copy of the top-k in registers.
