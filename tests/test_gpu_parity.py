@@ -319,6 +319,10 @@ SCREEN_CASES = [
     ("tiny-70", lambda rng: rng.random((70, 3)), 32),
     ("uniform-k2", lambda rng: rng.random((50_000, 2)), 32),
     ("clusters", lambda rng: np.concatenate([rng.normal(c, 0.01, (5000, 3)) for c in rng.random((8, 3))]), 16),
+    # 200 exact copies of each of 10 points: a query on a copy has more survivors at
+    # distance 0 in its first block than k, so the sorted first block stops at the k-th and
+    # the traversal's visit of the block takes the rest
+    ("hot-duplicates", lambda rng: np.concatenate([rng.random((20_000, 3)), np.repeat(rng.random((10, 3)), 200, axis=0)]), 32),
 ]
 
 
