@@ -129,13 +129,15 @@ __device__ __forceinline__ void periodic_traverse(const DevTree<A, K>& t, const 
     pending &= ~(1u << l);
     idx = (idx >> (level - l - 1)) ^ 1u;
     level = l + 1;
-    // rebuild the node's box, offsets and rd along its root path
+    // rebuild the node's box, offsets and rd along its root path. Level j splits dimension
+    // j % K and the path starts at level 0, so the levels go in groups of K with each
+    // level's dimension a constant (a run-time dimension costs a select per element of
+    // every vector it reads or writes).
     lo.fill(O(0));
     hi = pq.box;
     off.fill(O(0));
     rd = O(0);
-    int d = 0;
-    for (int j = 0; j < level; ++j) {
+    auto replay = [&](const int j, const int d) {
       const uint32_t node = idx >> (level - j);
       const bool side = ((idx >> (level - j - 1)) & 1u) != 0;
       const A pivot = stem_at(t, node);
@@ -149,8 +151,15 @@ __device__ __forceinline__ void periodic_traverse(const DevTree<A, K>& t, const 
         hi.set(d, n_hi);
         off.set(d, n_off);
       }
-      d = (d + 1 == K) ? 0 : d + 1;
+    };
+    int j = 0;
+    for (; j + K <= level; j += K) {
+#pragma unroll
+      for (int d = 0; d < K; ++d) replay(j + d, d);
     }
+#pragma unroll
+    for (int d = 0; d < K - 1; ++d)
+      if (j + d < level) replay(j + d, d);
   }
 }
 
