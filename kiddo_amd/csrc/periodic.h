@@ -81,42 +81,49 @@ __device__ __forceinline__ void periodic_traverse(const DevTree<A, K>& t, const 
   O rd = O(0);
   uint32_t idx = 1, pending = 0;
   int level = 0;
-  for (;;) {
-    bool reached = true;
-    int dim = level % K;
-    while (level < D) {
-      const A pivot = stem_at(t, idx);
-      c.stems += 1;
-      if (!(pivot < Ax<A>::maxv())) {  // sentinel stem: left child, state unchanged
-        idx <<= 1;
-        ++level;
-        dim = (dim + 1 == K) ? 0 : dim + 1;
-        continue;
-      }
-      const O pw = O(pivot), ol = lo[dim], ou = hi[dim];
-      const bool right = q[dim] >= pivot;
-      const O r_lo = ol < pw ? pw : ol;  // max(lower, pivot)
-      const O l_hi = ou < pw ? ou : pw;  // min(upper, pivot)
-      const O n_lo = right ? r_lo : ol, n_hi = right ? ou : l_hi;
-      const O f_lo = right ? ol : r_lo, f_hi = right ? l_hi : ou;
-      const O qd = pq.q[dim], L = pq.box[dim];
-      const O n_off = periodic_interval_dist(qd, n_lo, n_hi, L);
-      const O f_off = periodic_interval_dist(qd, f_lo, f_hi, L);
-      const O n_rd = pq.met.rect(rd, off, dim, n_off);
-      const O f_rd = pq.met.rect(rd, off, dim, f_off);
-      if (!vis.prune(f_rd)) pending |= 1u << level;
-      idx = (idx << 1) | (right ? 1u : 0u);
+  // One level on dimension d (= level % K); false when the near child is pruned (then pop).
+  auto descend = [&](const int d) -> bool {
+    const A pivot = stem_at(t, idx);
+    c.stems += 1;
+    if (!(pivot < Ax<A>::maxv())) {  // sentinel stem: left child, state unchanged
+      idx <<= 1;
       ++level;
-      if (vis.prune(n_rd)) {  // near child pruned: pop
-        reached = false;
-        break;
-      }
-      lo.set(dim, n_lo);
-      hi.set(dim, n_hi);
-      off.set(dim, n_off);
-      rd = n_rd;
-      dim = (dim + 1 == K) ? 0 : dim + 1;
+      return true;
     }
+    const O pw = O(pivot), ol = lo[d], ou = hi[d];
+    const bool right = q[d] >= pivot;
+    const O r_lo = ol < pw ? pw : ol;  // max(lower, pivot)
+    const O l_hi = ou < pw ? ou : pw;  // min(upper, pivot)
+    const O n_lo = right ? r_lo : ol, n_hi = right ? ou : l_hi;
+    const O f_lo = right ? ol : r_lo, f_hi = right ? l_hi : ou;
+    const O qd = pq.q[d], L = pq.box[d];
+    const O n_off = periodic_interval_dist(qd, n_lo, n_hi, L);
+    const O f_off = periodic_interval_dist(qd, f_lo, f_hi, L);
+    const O n_rd = pq.met.rect(rd, off, d, n_off);
+    const O f_rd = pq.met.rect(rd, off, d, f_off);
+    if (!vis.prune(f_rd)) pending |= 1u << level;
+    idx = (idx << 1) | (right ? 1u : 0u);
+    ++level;
+    if (vis.prune(n_rd)) return false;
+    lo.set(d, n_lo);
+    hi.set(d, n_hi);
+    off.set(d, n_off);
+    rd = n_rd;
+    return true;
+  };
+  for (;;) {
+    // levels one at a time up to a multiple of K (a run-time dimension), then groups of K
+    // levels whose dimensions are constants, then the rest
+    bool reached = true;
+    while (reached && level < D && level % K != 0) reached = descend(level % K);
+    while (reached && level + K <= D) {
+#pragma unroll
+      for (int d = 0; d < K; ++d)
+        if (reached) reached = descend(d);
+    }
+#pragma unroll
+    for (int d = 0; d < K - 1; ++d)
+      if (reached && level < D) reached = descend(d);
     if (reached) {
       const uint32_t leaf = idx - (1u << D);
       c.leaves += 1;
