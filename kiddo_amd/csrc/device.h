@@ -252,23 +252,33 @@ __device__ __forceinline__ void traverse(const DevTree<A, K>& t, const Small<A, 
     // descend to a leaf (orchestrator/mod.rs:520-603)
     c.stems += uint32_t(lim - level);
     const O bnd = vis.bound();  // the bound only changes in a leaf visit
-    // branch-free step: a sentinel stem (no pivot) sends the query left and queues nothing
-    auto step = [&](const A pivot) {
+    // branch-free step on dimension d (= level % K): a sentinel stem (no pivot) sends the
+    // query left and queues nothing
+    auto step = [&](const A pivot, const int d) {
       const bool valid = pivot < Ax<A>::maxv();
-      const bool right = valid && q[dim] >= pivot;
-      const O no = abs_diff(qw[dim], O(pivot));
-      const O rdf = met.rect(rd, off, dim, no);
+      const bool right = valid && q[d] >= pivot;
+      const O no = abs_diff(qw[d], O(pivot));
+      const O rdf = met.rect(rd, off, d, no);
       const bool push = valid && (!check || !(rdf > bnd));
       pending |= (push ? 1u : 0u) << level;
       idx = (idx << 1) | (right ? 1u : 0u);
       ++level;
-      dim = (dim + 1 == K) ? 0 : dim + 1;
     };
+    auto next_dim = [&]() { dim = (dim + 1 == K) ? 0 : dim + 1; };
     // the levels held in LDS, then the rest: each loop reads one kind of memory (a load
-    // that picks LDS or global memory per step needs a generic address built every step)
+    // that picks LDS or global memory per step needs a generic address built every step).
+    // The global levels go one at a time up to a multiple of K, then in groups of K whose
+    // dimensions are constants (a run-time dimension costs a select per vector element).
     const int mid = lim < lds_levels ? lim : lds_levels;
-    while (level < mid) step(t.lds[idx]);
-    while (level < lim) step(t.stems[idx]);
+    for (; level < mid; next_dim()) step(t.lds[idx], dim);
+    for (; level < lim && dim != 0; next_dim()) step(t.stems[idx], dim);
+    while (level + K <= lim) {
+#pragma unroll
+      for (int d = 0; d < K; ++d) step(t.stems[idx], d);
+    }
+#pragma unroll
+    for (int d = 0; d < K - 1; ++d)
+      if (level < lim) step(t.stems[idx], d);
     {
       const uint32_t nl = final ? 0u : 1u << (D - lim);
       idx <<= (D - lim);  // the block's first leaf: its ancestors above the block are unchanged
