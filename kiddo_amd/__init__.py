@@ -419,6 +419,18 @@ class ImmutableKdTree:
         dt = dtype or {np.float32: torch.float32, np.float64: torch.float64}[self.out_dtype]
         return torch.empty(shape, dtype=dt, device="cuda")
 
+    def _check_out(self, out, m, k):
+        """Caller-provided result buffers must be contiguous cuda tensors of exactly the
+        shapes and dtypes the kernel writes (it writes m x k items and distances, m counts)."""
+        torch = _torch()
+        odt = {np.float32: torch.float32, np.float64: torch.float64}[self.out_dtype]
+        for name, t, shape, dt in (("items", out.items, (m, k), torch.uint32),
+                                   ("distances", out.distances, (m, k), odt),
+                                   ("counts", out.counts, (m,), torch.uint32)):
+            if (not isinstance(t, torch.Tensor) or not t.is_cuda or tuple(t.shape) != shape
+                    or t.dtype != dt or not t.is_contiguous()):
+                raise ValueError(f"out.{name} must be a contiguous cuda {dt} tensor of shape {shape}")
+
     def nearest_one_raw(self, q, flags=0, stream=None, approx=False):
         torch = _torch()
         q = self._queries(q)
@@ -441,6 +453,8 @@ class ImmutableKdTree:
         if out is None:
             out = NearestN(torch.empty((m, k), dtype=torch.uint32, device="cuda"),
                            self._out(m, k), torch.empty(m, dtype=torch.uint32, device="cuda"))
+        else:
+            self._check_out(out, m, k)
         _check(lib().kd_nearest_n(self._h, q.data_ptr(), m, k, float(radius), flags,
                                   out.items.data_ptr(), out.distances.data_ptr(),
                                   out.counts.data_ptr(), _stream_ptr(stream)), "kd_nearest_n")
@@ -454,6 +468,8 @@ class ImmutableKdTree:
         sp = _stream_ptr(stream)
         _check(lib().kd_within_count(self._h, q.data_ptr(), m, float(radius), flags,
                                      offsets.data_ptr(), sp), "kd_within_count")
+        if stream is not None:
+            stream.synchronize()  # the total below is read on torch's current stream
         total = int(offsets[m].item()) if m else 0
         items = torch.empty(max(total, 1), dtype=torch.uint32, device="cuda")
         d = self._out(max(total, 1))
@@ -514,6 +530,8 @@ class ImmutableKdTree:
         _check(lib().kd_periodic_within_count(self._h, q.data_ptr(), m, _np_ptr(b), float(radius),
                                               flags, offsets.data_ptr(), sp),
                "kd_periodic_within_count")
+        if stream is not None:
+            stream.synchronize()  # the total below is read on torch's current stream
         total = int(offsets[m].item()) if m else 0
         items = torch.empty(max(total, 1), dtype=torch.uint32, device="cuda")
         d = self._out(max(total, 1))
@@ -597,6 +615,8 @@ class Forest:
         sp = _stream_ptr(stream)
         _check(lib().kd_forest_within(self._h, queries.data_ptr(), float(radius), flags,
                                       offsets.data_ptr(), None, None, sp), "kd_forest_within(count)")
+        if stream is not None:
+            stream.synchronize()  # the total below is read on torch's current stream
         total = int(offsets[self.m].item()) if self.m else 0
         dt = {np.float32: torch.float32, np.float64: torch.float64}[self.out_dtype]
         items = torch.empty(max(total, 1), dtype=torch.uint32, device="cuda")

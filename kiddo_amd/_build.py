@@ -7,6 +7,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -37,8 +38,17 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every translation unit in parallel, then links the shared library."""
+def _newest_input(src: str) -> float:
+    """Latest modification time of a source, the headers every unit includes, and this
+    file's flags."""
+    deps = [src, __file__, os.path.join(HERE, "..", "include", "kiddo_amd.h")]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    return max(os.path.getmtime(d) for d in deps if os.path.exists(d))
+
+
+def build(verbose: bool = False, force: bool = True) -> str:
+    """Compiles every translation unit in parallel, then links the shared library.
+    force=False recompiles only the units whose object is older than its inputs."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     os.makedirs(OBJ_DIR, exist_ok=True)
     procs = []
@@ -46,19 +56,25 @@ def build(verbose: bool = False) -> str:
     for src in sources():
         obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
         objs.append(obj)
+        if not force and os.path.exists(obj) and os.path.getmtime(obj) > _newest_input(src):
+            continue
         cmd = [hipcc, *HIP_FLAGS, "-c", "-o", obj, src]
         if src.endswith(".cpp"):
             cmd = [hipcc, "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", "-o", obj, src]
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                                            text=True)))
+        # each compiler writes to its own file: with pipes, a compiler whose output is not
+        # being read blocks once the pipe fills, and the parallel build runs one at a time
+        log = tempfile.TemporaryFile(mode="w+")
+        procs.append((src, log, subprocess.Popen(cmd, stdout=log, stderr=subprocess.STDOUT)))
     failed = []
-    for src, p in procs:
-        out, _ = p.communicate()
+    for src, log, p in procs:
+        p.wait()
         if p.returncode != 0:
             failed.append(src)
-            sys.stderr.write(out)
+            log.seek(0)
+            sys.stderr.write(log.read())
+        log.close()
     if failed:
         raise RuntimeError(f"hipcc failed on {failed}")
     tmp = LIB + ".tmp"
@@ -72,4 +88,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, force="--changed" not in sys.argv))

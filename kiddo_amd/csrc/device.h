@@ -1243,10 +1243,11 @@ template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {
           ok = cnt < k || less(d, it, kd, kit);
         }
       }
-      uint32_t mask = uint32_t((__ballot(ok) >> base) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull)));
+      // the group's bits of the 64-lane ballot (all 64 when G = 64)
+      uint64_t mask = (uint64_t(__ballot(ok)) >> base) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull));
       while (mask) {
-        const int src = __ffs(mask) - 1;
-        mask &= mask - 1u;
+        const int src = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1ull;
         const O cd = __shfl(d, src, G);
         const uint32_t ci = __shfl(it, src, G);
         if (cnt >= k && !less(cd, ci, kd, kit)) continue;

@@ -35,9 +35,11 @@ __device__ __forceinline__ void screen_setup(const Small<double, K>& q, const do
 // (the smallest block holding 4k entries, 64 at least for k >= 8; leaves under one stem
 // are contiguous) give a k-th smallest S32, and every one of those k entries has an FP64
 // distance of at most
-// (sqrt(S32 (1+4u)) + sqrt(K) eps)^2 (1 + 2e-15): the FP32 sum of non-negative squares is
-// at least ||delta||^2 (1-u)^K, each FP32 coordinate difference is within eps of the
-// exact one, and the FP64 evaluation is within (1 + 1e-15) of the exact square. Returns
+// (sqrt((S32 + 1e-44) (1+4u)) + sqrt(K) eps)^2 (1 + 2e-15): the FP32 sum of non-negative
+// squares is at least ||delta||^2 (1-u)^K - K 2^-150 (a product that underflows into the
+// subnormal range loses up to 2^-150 absolute, a sum with a subnormal result is exact;
+// 1e-44 > 3 * 2^-150), each FP32 coordinate difference is within eps of the exact one,
+// and the FP64 evaluation is within (1 + 1e-15) of the exact square. Returns
 // +inf when the tree holds fewer than k entries. `up` receives the block's height in
 // levels (2^up leaves) and [b, e) its entries; 0 and an empty range with an infinite cap.
 template <int K, int MAXK>
@@ -106,7 +108,7 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
   up_out = up;
   b_out = b;
   e_out = e;
-  const double r = sqrt(double(kth) * (1.0 + 4.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
+  const double r = sqrt((double(kth) + 1e-44) * (1.0 + 4.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
   return r * r * (1.0 + 2e-15);
 }
 

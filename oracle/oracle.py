@@ -259,6 +259,16 @@ class Tree:
                "within_fill")
         return (offsets, items, d, st) if stats else (offsets, items, d)
 
+    def within_count(self, q, radius, exclusive=False, threads=1):
+        """Per-query result counts of within(radius) (the count pass alone)."""
+        q = self._q(q)
+        m = q.shape[0]
+        counts = np.zeros(m, np.uint64)
+        st = np.zeros(3, np.uint64)
+        _check(lib().or_within_count(self._h, _p(q), m, float(radius), int(exclusive),
+                                     _p(counts), threads, _p(st)), "within_count")
+        return counts
+
     def set_metric(self, flags: int):
         """Metric of the following queries (flags bits 8-19, as the GPU engine's)."""
         _check(lib().or_set_metric(self._h, int(flags)), "set_metric")

@@ -2,9 +2,10 @@
   * the reference's metric tests (src/dist/mod.rs: distances 25, 7, 4, 91.125 and the
     Manhattan / Chebyshev rect-distance updates 9, 2, 6), from reference_kats.json;
   * query distances against an independent numpy evaluation in Kiddo's order;
-  * every family (nearest_one, nearest_n, nearest_n().within, within, best_n_within,
-    approx) against brute force, for f32, f64 and u16 trees, K = 2, 3, 4, which pins the
+  * every family (nearest_one, nearest_n, nearest_n().within, within, best_n_within)
+    against brute force, for f32, f64 and u16 trees, K = 2, 3, 4, which pins the
     traversal's rect updates (a wrong update prunes a subtree that holds an answer);
+    approx has no backtracking and is pinned separately (tests/test_oracle_approx.py);
   * periodic nearest_n / within under each metric against a minimum-image brute force.
 """
 import numpy as np
