@@ -180,6 +180,19 @@ int kd_approx_nearest_one(kd_tree* tree, const void* queries, uint64_t m, uint32
 int kd_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k, double radius,
                  uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream);
 
+/* kd_nearest_n that allocates nothing: no counterpart in the reference. It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and errors as kd_nearest_n;
+ * the Morton ordering of KD_MORTON runs in a scratch array the library holds in static
+ * device memory (32 MiB per GPU, kept until the process ends) instead of a per-call
+ * allocation. Limit: with KD_MORTON, m <= KD_NOALLOC_MAX_QUERIES (KD_ERR_CAPACITY
+ * beyond); without it any m (no scratch is used). The tree's device must be current.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU (on
+ * different streams, or from several threads) race; order them on one stream. */
+#define KD_NOALLOC_MAX_QUERIES (1u << 20)
+int kd_nearest_n_noalloc(kd_tree* tree, const void* queries, uint64_t m, uint32_t k, double radius,
+                         uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream);
+
 /* tree.query(q).within(radius)[.unsorted()][.exclusive_boundaries()], phase 1
  * (query/within.rs:18-41, within_unsorted.rs:133-156): counts results per query and
  * writes offsets[m+1] as their exclusive prefix sum (offsets[m] = total). */

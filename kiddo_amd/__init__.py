@@ -37,6 +37,8 @@ KD_MORTON = 1 << 2
 KD_SCREEN_F32 = 1 << 3
 KD_GRAPH = 1 << 4
 KD_TREE_SHADOW_F32 = 1 << 0
+# batch limit of kd_nearest_n_noalloc under KD_MORTON (include/kiddo_amd.h)
+KD_NOALLOC_MAX_QUERIES = 1 << 20
 # distance metrics (flag bits 8-19, include/kiddo_amd.h KD_METRIC_*)
 KD_METRIC_SQEUCLIDEAN = 0
 KD_METRIC_MANHATTAN = 1 << 8
@@ -104,6 +106,7 @@ SIGNATURES = {
     "kd_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_approx_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
+    "kd_nearest_n_noalloc": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_within_count": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp], C.c_int),
     "kd_within_fill": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_best_n_within": ([_vp, _vp, _u64, _f64, _u32, _u32, _vp, _vp, _vp, _vp], C.c_int),

@@ -344,8 +344,12 @@ struct Order {
   }
 };
 
+// ext (optional): caller-provided device scratch of ext_bytes used instead of a per-call
+// allocation (the keys and ids are written in full before the sort reads them, so it need
+// not be cleared).
 template <class A, int K>
-int make_order(const kd_tree* t, const A* queries, uint64_t m, uint32_t flags, hipStream_t s, Order& o) {
+int make_order(const kd_tree* t, const A* queries, uint64_t m, uint32_t flags, hipStream_t s, Order& o,
+               void* ext = nullptr, size_t ext_bytes = 0) {
   if (!(flags & KD_MORTON) || m < 2) return KD_OK;
   if (m > 0xFFFFFFFFull) return set_error(KD_ERR_INVALID, "Morton ordering supports < 2^32 queries");
   o.s = s;
@@ -354,8 +358,16 @@ int make_order(const kd_tree* t, const A* queries, uint64_t m, uint32_t flags, h
   KD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, k0, k1, v0, v1, int(m), 0, 32, s));
   const size_t arr = ((m * 4 + 255) / 256) * 256;
   const size_t total = 4 * arr + temp;
-  KD_HIP(hipMallocAsync(&o.mem, total, s));
-  uint8_t* base = static_cast<uint8_t*>(o.mem);
+  uint8_t* base = nullptr;
+  if (ext) {
+    if (total > ext_bytes)
+      return set_error(KD_ERR_CAPACITY, "Morton ordering of %llu queries needs %zu bytes of scratch, %zu held",
+                       (unsigned long long)m, total, ext_bytes);
+    base = static_cast<uint8_t*>(ext);
+  } else {
+    KD_HIP(hipMallocAsync(&o.mem, total, s));
+    base = static_cast<uint8_t*>(o.mem);
+  }
   k0 = reinterpret_cast<uint32_t*>(base);
   k1 = reinterpret_cast<uint32_t*>(base + arr);
   v0 = reinterpret_cast<uint32_t*>(base + 2 * arr);
@@ -379,6 +391,45 @@ int check_radius(double radius) {
   return KD_OK;
 }
 
+}  // namespace
+}  // namespace kd
+
+// ------------------------------------------------ allocation-free entry point --------
+// kd_nearest_n_noalloc takes its Morton-ordering scratch (keys and ids, two buffers each,
+// and the radix sort's temporary storage) from this array instead of a per-call
+// allocation: up to KD_NOALLOC_MAX_QUERIES queries, 32 MiB per GPU, held until the process
+// ends. Host code never passes `scratch` itself to a kernel or copy (on the host it only
+// names the symbol); it looks up each device's address once with hipGetSymbolAddress.
+namespace kd {
+namespace {
+constexpr size_t kNoAllocArr = size_t(KD_NOALLOC_MAX_QUERIES) * 4;  // one u32 array
+constexpr size_t kNoAllocSortTemp = size_t(16) << 20;  // checked per call against the sort's request
+constexpr size_t kScratchBytes = 4 * kNoAllocArr + kNoAllocSortTemp;
+}  // namespace
+}  // namespace kd
+
+static __device__ __align__(256) unsigned char scratch[kd::kScratchBytes];
+
+namespace kd {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+int device_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return set_error(KD_ERR_UNSUPPORTED, "device %d", device);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(KD_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return KD_OK;
+}
 }  // namespace
 }  // namespace kd
 
@@ -587,6 +638,39 @@ int kd_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double
   if (m == 0) return KD_OK;
   if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
   KD_ORDERED(a, queries, m, flags, stream);
+  return kd::launch_nearest_n(a, int(k), radius, items, dists, counts);
+}
+
+int kd_nearest_n_noalloc(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double radius,
+                         uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1 (NonZeroUsize)");
+  if (k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "k above %d is not provided", kd::kMaxK);
+  if (int rc = kd::check_radius(radius)) return rc;
+  if ((flags & KD_SCREEN_F32) && !t->shadow)
+    return kd::set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
+  if ((flags & KD_MORTON) && m > uint64_t(KD_NOALLOC_MAX_QUERIES))
+    return kd::set_error(KD_ERR_CAPACITY, "kd_nearest_n_noalloc orders at most %u queries (KD_MORTON), got %llu",
+                         unsigned(KD_NOALLOC_MAX_QUERIES), (unsigned long long)m);
+  if (m == 0) return KD_OK;
+  if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  if (int mrc = kd::check_metric(flags)) return mrc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  kd::Order order;
+  if ((flags & KD_MORTON) && m >= 2) {
+    int dev = 0;
+    KD_HIP(hipGetDevice(&dev));
+    if (dev != t->device)
+      return kd::set_error(KD_ERR_INVALID, "the tree lives on device %d, device %d is current", t->device, dev);
+    void* scr = nullptr;
+    if (int rc = kd::device_scratch(dev, &scr)) return rc;
+    int orc = KD_OK;
+    KD_DISPATCH_AK(t->dtype, t->dims,
+                   orc = (kd::make_order<A, KK>(t, static_cast<const A*>(queries), m, flags, s, order, scr,
+                                                kd::kScratchBytes)));
+    if (orc) return orc;
+  }
+  const kd::QueryArgs a{t, queries, m, order.perm, flags, s};
   return kd::launch_nearest_n(a, int(k), radius, items, dists, counts);
 }
 
