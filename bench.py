@@ -7,14 +7,22 @@ points (ties), bucket 32; 16,000,000 uniform queries; nearest_n(k=16). Inputs ar
 and synthetic and already resident in HBM when timing starts.
 
 A step is one batch of all queries through one kernel variant:
-  * plain   — the plain FP64 kernel, queries in input order;
-  * screened — the batched FP32-screened kernel: queries in Morton order, FP32 shadow
-               screening against a certified bound, FP64 rescore (identical results).
-`value` is the screened variant's queries/s. Also reported: the plain variant, their
-ratio, p50/p99 latency of fixed 262,144-query sub-batches, the fraction of scanned entries
-rescored in FP64 and the largest FP32 screen error seen, the HBM roofline of the screened
-kernel from its algorithmic bytes, and the CPU baseline (the oracle's restatement of
-Kiddo's scalar query, run on this host's cores over a sample of the same queries).
+  * plain       — the plain FP64 kernel, queries in Morton order, launch shape autotuned
+                  on the batch (thread or group per query, LDS stem levels, occupancy);
+  * plain_input — the plain FP64 kernel in input order, one thread per query, no LDS stems;
+  * screened    — the batched FP32-screened kernel: queries in Morton order, FP32 shadow
+                  screening against a certified bound, FP64 rescore (identical results),
+                  launch shape autotuned the same way.
+`value` is the screened variant's queries/s; `screened_over_plain` compares it with the
+plain kernel under the same query order and tuning (like for like), and
+`screened_over_plain_input_order` with the plain kernel in input order. All three
+variants must agree bit for bit, and a sample of 200,000 queries spread over the batch is
+compared bit for bit (items, order, distances, counts) with the oracle in its canonical
+mode: the bench exits with status 3 on any mismatch. Also reported: p50/p99 latency of
+fixed 262,144-query sub-batches, the fraction of scanned entries rescored in FP64 and the
+largest FP32 screen error seen, the HBM roofline of the screened kernel from its
+algorithmic bytes, and the CPU baseline (the oracle's restatement of Kiddo's scalar query,
+run on this host's cores over a sample of the same queries).
 
     python bench.py [--steps 5] [--warmup 2]
 """
@@ -55,7 +63,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["both", "plain", "screened"], default="both")
+    ap.add_argument("--only", choices=["all", "plain", "plain_input", "screened"], default="all")
+    ap.add_argument("--parity-sample", type=int, default=200_000,
+                    help="knn16: queries compared with the canonical oracle (0: none)")
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--no-within", action="store_true", help="knn16: skip the within(r=0.002) pass")
     ap.add_argument("--no-device-build", action="store_true", help="knn16: skip timing the GPU tree build")
@@ -125,19 +135,20 @@ def main():
 
     m, k = args.queries, args.k
     q = torch.from_numpy(qh).to(dev)
+    variants = ["plain_input", "plain", "screened"] if args.only == "all" else [args.only]
     outs = {name: ka.NearestN(torch.empty((m, k), dtype=torch.uint32, device=dev),
                               torch.empty((m, k), dtype=torch.float64, device=dev),
                               torch.empty(m, dtype=torch.uint32, device=dev))
-            for name in ("plain", "screened")}
-    flags = {"plain": 0, "screened": ka.KD_SCREEN_F32 | ka.KD_MORTON}
-    variants = ["plain", "screened"] if args.only == "both" else [args.only]
+            for name in variants}
+    flags = {"plain_input": 0, "plain": ka.KD_MORTON, "screened": ka.KD_SCREEN_F32 | ka.KD_MORTON}
 
-    # plain: one thread per query, queries in input order, no LDS stems. screened: the
-    # launch shape the autotuner picks on a sample of the batch.
-    tuning = {"plain": dict(variant=1, group_size=0, lds_levels=0, waves=0)}
-    if "screened" in variants:
-        tuning["screened"] = tree.autotune_nearest_n(q, k, math.inf, flags["screened"])
-        log(f"autotuned screened launch: {tuning['screened']}")
+    # plain_input: one thread per query, input order, no LDS stems. plain and screened: the
+    # launch shape the autotuner picks for each on a sample of the batch.
+    tuning = {"plain_input": dict(variant=1, group_size=0, lds_levels=0, waves=0)}
+    for name in ("plain", "screened"):
+        if name in variants:
+            tuning[name] = tree.autotune_nearest_n(q, k, math.inf, flags[name])
+            log(f"autotuned {name} launch: {tuning[name]}")
 
     def run(name, qq=q, out=None):
         tree.nearest_n_raw(qq, k, math.inf, flags[name], out=out or outs[name])
@@ -161,13 +172,36 @@ def main():
         log(f"{name}: {results[name]['ms_per_step']:.1f} ms/step, "
             f"{results[name]['qps'] / 1e6:.1f} Mq/s, kernel {results[name]['kernel_ms']:.1f} ms, {stats}")
 
+    # every variant returns the same results, bit for bit
     parity = None
-    if len(variants) == 2:
-        parity = bool(torch.equal(outs["plain"].items, outs["screened"].items) and
-                      torch.equal(outs["plain"].distances.view(torch.int64),
-                                  outs["screened"].distances.view(torch.int64)))
-        log(f"screened == plain bit for bit: {parity}")
-    head = "screened" if "screened" in results else "plain"
+    if len(variants) > 1:
+        a = outs[variants[0]]
+        parity = all(bool(torch.equal(a.items, outs[v].items) and torch.equal(a.counts, outs[v].counts) and
+                          torch.equal(a.distances.view(torch.int64), outs[v].distances.view(torch.int64)))
+                     for v in variants[1:])
+        log(f"{' == '.join(variants)} bit for bit: {parity}")
+    head = "screened" if "screened" in results else variants[-1]
+
+    # a sample spread over the batch against the oracle's canonical answers
+    oracle_parity = None
+    if args.parity_sample:
+        from oracle import oracle as O
+        O.build()
+        ot_c = O.Tree.from_layout(np.float64, 3, args.bucket, pts.shape[0], lay.max_stem_level,
+                                  lay.stems(), lay.extents(), lay.arena())
+        ns = min(args.parity_sample, m)
+        rows = np.sort(np.random.default_rng(args.seed + 5).choice(m, ns, replace=False))
+        ref = ot_c.nearest_n(np.ascontiguousarray(qh[rows]), k, mode=O.CANONICAL,
+                             threads=min(16, os.cpu_count() or 1))
+        del ot_c
+        ridx = torch.from_numpy(rows).to(dev)
+        oracle_parity = {}
+        for name in variants:
+            got = (outs[name].items.view(torch.int32)[ridx].cpu().numpy().view(np.uint32),
+                   outs[name].distances[ridx].cpu().numpy(), outs[name].counts.view(torch.int32)[ridx].cpu().numpy().view(np.uint32))
+            oracle_parity[name] = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[2], ref[2]) and
+                                       np.array_equal(got[1].view(np.uint64), ref[1].view(np.uint64)))
+        log(f"oracle (canonical) on {ns} sampled queries: {oracle_parity}")
 
     # latency of fixed-size sub-batches (screened path)
     tree.tune(**{("knn_variant" if a == "variant" else a): v for a, v in tuning[head].items()})
@@ -209,7 +243,7 @@ def main():
     # visits exactly the leaves and stems Kiddo's traversal does; the screened kernel starts
     # from a certified bound and visits fewer): every scanned entry (K*8 coordinate bytes +
     # 4 item bytes), every stem read, the queries and the results
-    st = results["plain"]["stats"] if "plain" in results else results[head]["stats"]
+    st = results["plain_input"]["stats"] if "plain_input" in results else results[head]["stats"]
     own = results[head]["stats"]
     algo_bytes = st["points"] * (3 * 8 + 4) + st["stems"] * 8 + m * 3 * 8 + m * k * (4 + 8)
     kern_s = results[head]["kernel_ms"] / 1e3
@@ -309,9 +343,17 @@ def main():
                    "variant": "FP32-screened, Morton-ordered" if head == "screened" else "plain FP64",
                    "launch": tuning[head]},
         "plain_f64_qps": round(results["plain"]["qps"], 1) if "plain" in results else None,
+        "plain_f64_launch": tuning.get("plain"),
+        "plain_f64_input_order_qps": (round(results["plain_input"]["qps"], 1)
+                                      if "plain_input" in results else None),
         "screened_over_plain": (round(results["screened"]["qps"] / results["plain"]["qps"], 3)
-                                if len(results) == 2 else None),
-        "screened_equals_plain": parity,
+                                if "screened" in results and "plain" in results else None),
+        "screened_over_plain_input_order": (round(results["screened"]["qps"] / results["plain_input"]["qps"], 3)
+                                            if "screened" in results and "plain_input" in results else None),
+        "kernel_ms": {name: round(results[name]["kernel_ms"], 3) for name in results},
+        "variants_equal": parity,
+        "oracle_parity_sample": ({"queries": min(args.parity_sample, m), "mode": "canonical",
+                                  "equal": oracle_parity} if oracle_parity is not None else None),
         "p50_ms": p50, "p99_ms": p99, "latency_batch": lb,
         "fraction_rescored": (round(own["rescored"] / max(own["points"], 1), 4)
                               if head == "screened" else None),
@@ -327,6 +369,9 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if parity is False or (oracle_parity is not None and not all(oracle_parity.values())):
+        log("results differ between variants or from the oracle")
+        sys.exit(3)
 
 
 def bench_forest(args, torch, ka):
