@@ -631,7 +631,6 @@ int kd_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double
                  uint32_t* items, void* dists, uint32_t* counts, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;
   if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1 (NonZeroUsize)");
-  if (k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "k above %d is not provided", kd::kMaxK);
   if (int rc = kd::check_radius(radius)) return rc;
   if ((flags & KD_SCREEN_F32) && !t->shadow)
     return kd::set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
@@ -645,7 +644,6 @@ int kd_nearest_n_noalloc(kd_tree* t, const void* queries, uint64_t m, uint32_t k
                          uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;
   if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1 (NonZeroUsize)");
-  if (k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "k above %d is not provided", kd::kMaxK);
   if (int rc = kd::check_radius(radius)) return rc;
   if ((flags & KD_SCREEN_F32) && !t->shadow)
     return kd::set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
@@ -728,7 +726,6 @@ int kd_best_n_within(kd_tree* t, const void* queries, uint64_t m, double radius,
                      uint32_t* items, void* dists, uint32_t* counts, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;
   if (n == 0) return kd::set_error(KD_ERR_INVALID, "n must be >= 1 (NonZero)");
-  if (n > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "n above %d is not provided", kd::kMaxK);
   if (int rc = kd::check_radius(radius)) return rc;
   if (m == 0) return KD_OK;
   if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
@@ -769,7 +766,6 @@ int kd_periodic_nearest_n(kd_tree* t, const void* queries, uint64_t m, const voi
   if (int rc = kd::check_tree(t)) return rc;
   if (int rc = kd::check_periodic(t, box)) return rc;
   if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1 (NonZeroUsize)");
-  if (k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_UNSUPPORTED, "k above %d is not provided", kd::kMaxK);
   if (int rc = kd::check_radius(radius)) return rc;
   if (m == 0) return KD_OK;
   if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
@@ -825,7 +821,7 @@ int kd_periodic_within_fill(kd_tree* t, const void* queries, uint64_t m, const v
 int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double radius,
                           uint32_t flags, int32_t* chosen4, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;
-  if (k == 0 || k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_INVALID, "k out of range");
+  if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1");
   if (!queries) return kd::set_error(KD_ERR_INVALID, "queries is null");
   hipStream_t s = static_cast<hipStream_t>(stream);
   // 2^20 queries: a thread-per-query candidate then still fills the device (256 CUs x 2048

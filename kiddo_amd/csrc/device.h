@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace kd {
 namespace dev {
@@ -693,6 +694,190 @@ __device__ __forceinline__ float screen_threshold(double T, double eps_k) {
   if ((double)f < tt) f = next_up(f);
   return f;
 }
+
+// Per-query FP32 screen set-up: the query rounded to FP32, and sqrt(K) times the bound on
+// the error of an FP32 coordinate difference (see screen_threshold).
+template <int K>
+__device__ __forceinline__ void screen_setup(const Small<double, K>& q, const double* __restrict__ max_abs,
+                                             float (&qf)[3], double& eps_k) {
+  double amax = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) qf[d] = 0.0f;
+#pragma unroll
+  for (int d = 0; d < K; ++d) {
+    qf[d] = (float)q[d];
+    amax = fmax(amax, max_abs[d] + fabs(q[d]));
+  }
+  const double u = 0x1p-24;
+  const double eps = u * (2.0 + u) * amax + 3.0 * 0x1p-150;
+  eps_k = sqrt((double)K) * eps * (1.0 + 1e-6);
+}
+
+// ------------------------------------------------ collectors for large k or n --------
+// Above the register forms (k or n > 64) the collector is a binary max-heap held in the
+// query's own output row (k entries of HBM, cached in L2), as Kiddo's BinaryHeap collector
+// for large k (result_collection.rs:365-461, BinaryHeap of BestQueryResultItem for
+// best_n_within, :463-561). The root is the largest kept entry, copied in registers: it
+// is the bound once the heap is full. A full heap takes a candidate that is smaller than
+// the root (replace the root, sift down); a heap not yet full appends and sifts up. The
+// row is sorted in place when the traversal ends (sort_segment) and padded to k.
+// Order: Key::less(a, b) over (distance, item) entries; the heap keeps the k smallest.
+struct KnnKey {  // nearest_n: (distance, then lower item)
+  template <class O> static __device__ __forceinline__ bool less(O ad, uint32_t ai, O bd, uint32_t bi) {
+    return ad < bd || (ad == bd && ai < bi);
+  }
+};
+struct ItemKey {  // best_n_within: item only (best_query_result_item.rs:27-31)
+  template <class O> static __device__ __forceinline__ bool less(O, uint32_t ai, O, uint32_t bi) { return ai < bi; }
+};
+
+template <class O, class Key> struct RowHeap {
+  uint32_t* __restrict__ it;
+  O* __restrict__ d;
+  uint32_t cap, cnt;
+  O rd;          // root (largest kept entry) once cnt > 0
+  uint32_t rit;
+  __device__ void init(uint32_t* it_, O* d_, uint32_t cap_) {
+    it = it_;
+    d = d_;
+    cap = cap_;
+    cnt = 0;
+    rd = oinf<O>();
+    rit = 0xFFFFFFFFu;
+  }
+  __device__ bool full() const { return cnt >= cap; }
+  // would the heap keep (cd, ci)?
+  __device__ bool takes(O cd, uint32_t ci) const { return !full() || Key::less(cd, ci, rd, rit); }
+  // places (cd, ci) at slot j of the heap [0, n) whose children below j are heaps
+  __device__ void sift_down(uint32_t j, uint32_t n, O cd, uint32_t ci) {
+    for (;;) {
+      const uint32_t c = 2 * j + 1;
+      if (c >= n) break;
+      O bd = d[c];
+      uint32_t bi = it[c], b = c;
+      if (c + 1 < n) {
+        const O d2 = d[c + 1];
+        const uint32_t i2 = it[c + 1];
+        if (Key::less(bd, bi, d2, i2)) {
+          bd = d2;
+          bi = i2;
+          b = c + 1;
+        }
+      }
+      if (!Key::less(cd, ci, bd, bi)) break;
+      d[j] = bd;
+      it[j] = bi;
+      j = b;
+    }
+    d[j] = cd;
+    it[j] = ci;
+  }
+  // requires takes(cd, ci)
+  __device__ void add(O cd, uint32_t ci) {
+    if (!full()) {
+      uint32_t j = cnt++;
+      while (j > 0) {  // sift up
+        const uint32_t p = (j - 1) >> 1;
+        const O pd = d[p];
+        const uint32_t pi = it[p];
+        if (!Key::less(pd, pi, cd, ci)) break;
+        d[j] = pd;
+        it[j] = pi;
+        j = p;
+      }
+      d[j] = cd;
+      it[j] = ci;
+    } else {
+      sift_down(0, cnt, cd, ci);  // replaces the root
+    }
+    rd = d[0];
+    rit = it[0];
+  }
+  // Heapsort's extraction phase (the row is a max-heap already): the kept entries end up
+  // ascending in Key order; the row is padded to cap with (item 0xFFFFFFFF, +inf).
+  __device__ void finish() {
+    for (uint32_t e = cnt; e > 1; --e) {
+      const O td = d[e - 1];
+      const uint32_t ti = it[e - 1];
+      d[e - 1] = d[0];
+      it[e - 1] = it[0];
+      sift_down(0, e - 1, td, ti);
+    }
+    for (uint32_t j = cnt; j < cap; ++j) {
+      it[j] = 0xFFFFFFFFu;
+      d[j] = oinf<O>();
+    }
+  }
+};
+
+// nearest_n / nearest_n().within for k above the register forms: the rules of KnnVis with
+// the top-k in a RowHeap. SCREEN (f64 trees with the FP32 shadow): an entry is measured in
+// FP64 only when its FP32 distance passes the certified threshold of the current bound
+// (screen_threshold), so the result is the plain one.
+template <class O, bool SCREEN> struct HeapKnnVis {
+  RowHeap<O, KnnKey> h;
+  O radius;
+  bool excl;
+  float qf[3];   // SCREEN: the query in FP32
+  double eps_k;  // SCREEN: sqrt(K) * per-query coordinate-difference error bound
+  float thr;     // SCREEN: FP32 threshold of the current bound
+  uint32_t rescored;
+  __device__ O bound() const { return (h.full() && h.rd < radius) ? h.rd : radius; }
+  __device__ bool prune(O rd) const { return rd > bound() || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
+    for (uint32_t p = s; p < e; ++p) {
+      if constexpr (SCREEN) {
+        const float4 sh = t.shadow[p];
+        const float dx = sub_rn(sh.x, qf[0]);
+        float acc = mul_rn(dx, dx);
+        const float dy = sub_rn(sh.y, qf[1]);
+        acc = add_rn(acc, mul_rn(dy, dy));
+        if (K > 2) {
+          const float dz = sub_rn(sh.z, qf[2]);
+          acc = add_rn(acc, mul_rn(dz, dz));
+        }
+        if (acc > thr) continue;
+        ++rescored;
+      }
+      const O d = point_dist(t, p, qw);
+      if (!(excl ? d < radius : d <= radius)) continue;
+      if (h.full() && !(d <= h.rd)) continue;
+      const uint32_t it = t.items[p];
+      if (!h.takes(d, it)) continue;
+      h.add(d, it);
+      if constexpr (SCREEN) {
+        if (h.full()) thr = screen_threshold(double(bound()), eps_k);
+      }
+    }
+  }
+};
+
+// best_n_within for n above the register form: the n smallest items within the radius
+// (best_n_within.rs:163-196) in a RowHeap ordered by item.
+template <class O> struct HeapBestNVis {
+  RowHeap<O, ItemKey> h;
+  O radius;
+  bool excl;
+  __device__ O bound() const { return radius; }
+  __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
+    for (uint32_t p = s; p < e; ++p) {
+      const uint32_t it = t.items[p];
+      if (!h.takes(O(0), it)) continue;
+      const O d = point_dist(t, p, qw);
+      if (!(excl ? d < radius : d <= radius)) continue;
+      h.add(d, it);
+    }
+  }
+};
 
 // A bit set of W*64 entries of a visit, taken lowest first (constant word indices only, so
 // it stays in registers).

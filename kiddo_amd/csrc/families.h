@@ -116,6 +116,64 @@ __global__ __launch_bounds__(kBlock) void k_best_n_within(DevTree<A, K> t, const
   flush_counters(cnt, stats);
 }
 
+// nearest_n / nearest_n().within for k above the register forms (k > 64): the top-k is a
+// max-heap in the query's output row (device.h RowHeap), sorted in place at the end.
+// SCREEN: f64 trees with K <= 3 and the FP32 shadow (SquaredEuclidean only).
+template <class A, int K, class M, bool SCREEN>
+__global__ __launch_bounds__(kBlock) void k_nearest_n_heap(DevTree<A, K> t, const A* __restrict__ queries,
+                                                           const uint32_t* __restrict__ order, uint64_t m, M met,
+                                                           int k, typename Ax<A>::O radius, int excl,
+                                                           const double* __restrict__ max_abs,
+                                                           uint32_t* __restrict__ out_items,
+                                                           typename Ax<A>::O* __restrict__ out_d,
+                                                           uint32_t* __restrict__ out_counts,
+                                                           unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    HeapKnnVis<O, SCREEN> v;
+    v.h.init(out_items + qi * uint64_t(k), out_d + qi * uint64_t(k), uint32_t(k));
+    v.radius = radius;
+    v.excl = excl != 0;
+    v.rescored = 0;
+    v.thr = __builtin_huge_valf();
+    if constexpr (SCREEN) {
+      static_assert(std::is_same<A, double>::value && K <= 3, "the FP32 screen is for f64 trees, K <= 3");
+      screen_setup<K>(q, max_abs, v.qf, v.eps_k);
+      v.thr = screen_threshold(double(v.bound()), v.eps_k);
+    }
+    traverse(t, q, mq, v, cnt);
+    v.h.finish();
+    out_counts[qi] = v.h.cnt;
+    cnt.rescored = v.rescored;
+  }
+  flush_counters(cnt, stats);
+}
+
+// best_n_within for n above the register form (n > 64): the n smallest items in a max-heap
+// by item in the query's output row, sorted by item at the end.
+template <class A, int K, class M>
+__global__ __launch_bounds__(kBlock) void k_best_n_within_heap(DevTree<A, K> t, const A* __restrict__ queries,
+                                                               const uint32_t* __restrict__ order, uint64_t m,
+                                                               M met, int n, typename Ax<A>::O radius, int excl,
+                                                               uint32_t* __restrict__ out_items,
+                                                               typename Ax<A>::O* __restrict__ out_d,
+                                                               uint32_t* __restrict__ out_counts,
+                                                               unsigned long long* __restrict__ stats) {
+  KD_QUERY_PROLOGUE
+  if (active) {
+    const auto mq = wrap_qw(qw, met);
+    HeapBestNVis<O> v;
+    v.h.init(out_items + qi * uint64_t(n), out_d + qi * uint64_t(n), uint32_t(n));
+    v.radius = radius;
+    v.excl = excl != 0;
+    traverse(t, q, mq, v, cnt);
+    v.h.finish();
+    out_counts[qi] = v.h.cnt;
+  }
+  flush_counters(cnt, stats);
+}
+
 template <class A, int K, class M>
 __global__ __launch_bounds__(kBlock) void k_within_count(DevTree<A, K> t, const A* __restrict__ queries,
                                                          const uint32_t* __restrict__ order, uint64_t m, M met,

@@ -12,24 +12,6 @@
 namespace kd {
 namespace dev {
 
-// Per-query FP32 screen set-up: the query rounded to FP32, and sqrt(K) times the bound on
-// the error of an FP32 coordinate difference (see screen_threshold).
-template <int K>
-__device__ __forceinline__ void screen_setup(const Small<double, K>& q, const double* __restrict__ max_abs,
-                                             float (&qf)[3], double& eps_k) {
-  double amax = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) qf[d] = 0.0f;
-#pragma unroll
-  for (int d = 0; d < K; ++d) {
-    qf[d] = (float)q[d];
-    amax = fmax(amax, max_abs[d] + fabs(q[d]));
-  }
-  const double u = 0x1p-24;
-  const double eps = u * (2.0 + u) * amax + 3.0 * 0x1p-150;
-  eps_k = sqrt((double)K) * eps * (1.0 + 1e-6);
-}
-
 // Certified upper bound on the k-th smallest FP64 distance from q, found before the
 // traversal: the FP32 distances of the entries of the query's own block of 2^j leaves
 // (the smallest block holding 4k entries, 64 at least for k >= 8; leaves under one stem
@@ -269,6 +251,12 @@ int launch_typed(const QueryArgs& a, int k, double radius, uint32_t* items, void
       return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 applies to f64 trees with dims <= 3");
     if (!t->shadow)
       return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
+  }
+  // above the register forms: the top-k in a heap in each query's output row
+  if (k > kMaxK) {
+    if constexpr (std::is_same<A, double>::value && K <= 3)
+      if (a.flags & KD_SCREEN_F32) return mlaunch_knn_heap<A, K, dev::SqEuclidean, true>(a, {}, k, radius, items, dists, counts);
+    return mlaunch_knn_heap<A, K, dev::SqEuclidean, false>(a, {}, k, radius, items, dists, counts);
   }
   // group form: k must fit in one slot per lane
   int g = t->group_size;

@@ -164,6 +164,32 @@ __global__ __launch_bounds__(kBlock) void k_periodic_nearest_n(DevTree<A, K> t, 
   flush_counters(cnt, stats);
 }
 
+// periodic nearest_n for k above the register forms: the top-k in a max-heap in the
+// query's output row (device.h RowHeap), sorted in place at the end
+template <class A, int K, class M>
+__global__ __launch_bounds__(kBlock) void k_periodic_nearest_n_heap(DevTree<A, K> t, const A* __restrict__ queries,
+                                                                    const uint32_t* __restrict__ order, uint64_t m,
+                                                                    M met, PBox<A> box, int k,
+                                                                    typename Ax<A>::O radius, int excl,
+                                                                    uint32_t* __restrict__ out_items,
+                                                                    typename Ax<A>::O* __restrict__ out_d,
+                                                                    uint32_t* __restrict__ out_counts,
+                                                                    unsigned long long* __restrict__ stats) {
+  KD_PERIODIC_PROLOGUE
+  if (active) {
+    HeapKnnVis<O, false> v;
+    v.h.init(out_items + qi * uint64_t(k), out_d + qi * uint64_t(k), uint32_t(k));
+    v.radius = radius;
+    v.excl = excl != 0;
+    v.rescored = 0;
+    v.thr = __builtin_huge_valf();
+    periodic_traverse(t, q, pq, v, cnt);
+    v.h.finish();
+    out_counts[qi] = v.h.cnt;
+  }
+  flush_counters(cnt, stats);
+}
+
 template <class A, int K, class M>
 __global__ __launch_bounds__(kBlock) void k_periodic_within_count(DevTree<A, K> t, const A* __restrict__ queries,
                                                                   const uint32_t* __restrict__ order, uint64_t m,
@@ -243,11 +269,29 @@ int mlaunch_knn_typed(const QueryArgs& a, M met, int k, double radius, uint32_t*
   return KD_OK;
 }
 
+// nearest_n for k above the register forms (any metric; SCREEN: SquaredEuclidean f64 trees
+// with K <= 3 and the FP32 shadow)
+template <class A, int K, class M, bool SCREEN>
+int mlaunch_knn_heap(const QueryArgs& a, M met, int k, double radius, uint32_t* items, void* dists,
+                     uint32_t* counts) {
+  using O = typename dev::Ax<A>::O;
+  auto dt = dev_tree<A, K>(a.t);
+  reset_counters(a.t, a.s);
+  KernelTimer kt(a.t, a.s);
+  hipLaunchKernelGGL((dev::k_nearest_n_heap<A, K, M, SCREEN>), dim3(grid_for(a.m)), dim3(dev::kBlock),
+                     lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, k, O(radius),
+                     (a.flags & KD_EXCLUSIVE) ? 1 : 0, a.t->d_max_abs, items, static_cast<O*>(dists), counts,
+                     a.t->counters);
+  KD_HIP(hipGetLastError());
+  return KD_OK;
+}
+
 // nearest_n with a metric other than SquaredEuclidean: the thread-per-query kernel
 template <class M>
 int mlaunch_nearest_n(const QueryArgs& a, M met, int k, double radius, uint32_t* items, void* dists,
                       uint32_t* counts) {
   KD_DISPATCH_AK(a.t->dtype, a.t->dims, {
+    if (k > kMaxK) return mlaunch_knn_heap<A, KK, M, false>(a, met, k, radius, items, dists, counts);
     if (k <= 1) return mlaunch_knn_typed<A, KK, 1>(a, met, k, radius, items, dists, counts);
     if (k <= 8) return mlaunch_knn_typed<A, KK, 8>(a, met, k, radius, items, dists, counts);
     if (k <= 16) return mlaunch_knn_typed<A, KK, 16>(a, met, k, radius, items, dists, counts);
@@ -274,6 +318,18 @@ template <class M>
 int mlaunch_best_n_within(const QueryArgs& a, M met, int n, double radius, uint32_t* items, void* dists,
                           uint32_t* counts) {
   KD_DISPATCH_AK(a.t->dtype, a.t->dims, {
+    if (n > kMaxK) {  // the n smallest items in a heap in the output row
+      using O = typename dev::Ax<A>::O;
+      auto dt = dev_tree<A, KK>(a.t);
+      reset_counters(a.t, a.s);
+      KernelTimer kt(a.t, a.s);
+      hipLaunchKernelGGL((dev::k_best_n_within_heap<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock),
+                         lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, n,
+                         O(radius), (a.flags & KD_EXCLUSIVE) ? 1 : 0, items, static_cast<O*>(dists), counts,
+                         a.t->counters);
+      KD_HIP(hipGetLastError());
+      return KD_OK;
+    }
     if (n <= 4) return mlaunch_best_typed<A, KK, 4>(a, met, n, radius, items, dists, counts);
     if (n <= 8) return mlaunch_best_typed<A, KK, 8>(a, met, n, radius, items, dists, counts);
     if (n <= 16) return mlaunch_best_typed<A, KK, 16>(a, met, n, radius, items, dists, counts);
@@ -356,6 +412,18 @@ template <class M>
 int mlaunch_periodic_nearest_n(const QueryArgs& a, M met, const void* box, int k, double radius, uint32_t* items,
                                void* dists, uint32_t* counts) {
   KD_DISPATCH_FK(a.t->dtype, a.t->dims, {
+    if (k > kMaxK) {  // the top-k in a heap in the output row
+      using O = typename dev::Ax<A>::O;
+      auto dt = dev_tree<A, KK>(a.t);
+      reset_counters(a.t, a.s);
+      KernelTimer kt(a.t, a.s);
+      hipLaunchKernelGGL((dev::k_periodic_nearest_n_heap<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock),
+                         lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met,
+                         pbox_of<A, KK>(box), k, O(radius), (a.flags & KD_EXCLUSIVE) ? 1 : 0, items,
+                         static_cast<O*>(dists), counts, a.t->counters);
+      KD_HIP(hipGetLastError());
+      return KD_OK;
+    }
     if (k <= 1) return mlaunch_pknn_typed<A, KK, 1>(a, met, box, k, radius, items, dists, counts);
     if (k <= 4) return mlaunch_pknn_typed<A, KK, 4>(a, met, box, k, radius, items, dists, counts);
     if (k <= 8) return mlaunch_pknn_typed<A, KK, 8>(a, met, box, k, radius, items, dists, counts);

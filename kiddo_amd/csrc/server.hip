@@ -345,8 +345,8 @@ int kd_server_submit(kd_server* S, const kd_request* r, uint64_t* ticket) {
   if (r->tree < 0 || uint32_t(r->tree) >= S->trees.size()) return kd::set_error(KD_ERR_INVALID, "bad tree index");
   if (r->m == 0 || !r->queries) return kd::set_error(KD_ERR_INVALID, "empty request");
   if ((r->family == KD_FAMILY_NEAREST_N || r->family == KD_FAMILY_BEST_N_WITHIN) &&
-      (r->k == 0 || r->k > uint32_t(kd::kMaxK)))
-    return kd::set_error(KD_ERR_INVALID, "k out of range");
+      r->k == 0)
+    return kd::set_error(KD_ERR_INVALID, "k must be >= 1");
   if (r->family < KD_FAMILY_NEAREST_ONE || r->family > KD_FAMILY_APPROX)
     return kd::set_error(KD_ERR_UNSUPPORTED, "family %d is not served", r->family);
   if (kd::launch_bytes(S->trees[r->tree], *r, r->m) > S->arena.size)
