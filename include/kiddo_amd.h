@@ -117,6 +117,17 @@ int kd_build_layout(int32_t dtype, int32_t dims, int32_t bucket, uint64_t n,
 int kd_layout_desc(const kd_layout* layout, kd_tree_desc* out);
 void kd_layout_free(kd_layout* layout);
 
+/* A KdTree with Donnelly<block_height> stems (kiddo src/stem_strategy/donnelly/core.rs:
+ * stems in lines of 2^block_height slots, each a triangle of block_height levels; the
+ * stem depth padded to a multiple of block_height with pivot-less levels above the real
+ * root, construction.rs:431-444), re-laid out on the host in the Eytzinger layout that
+ * kd_tree_create takes: the same pivot at every node of the same level, the same leaves
+ * (the empty leaves under the padding levels appended), so every query answers exactly as
+ * on the Donnelly tree. `donnelly` is the tree as a Rust shim reads it through
+ * KdTreeAccessor (src/traits/kd_tree.rs:16-46): stems in Donnelly order, leaf_count real
+ * leaves, max_stem_level including the padding. Free with kd_layout_free. */
+int kd_layout_from_donnelly(const kd_tree_desc* donnelly, int32_t block_height, kd_layout** out);
+
 /* ----------------------------------------------------------------- device trees ---- */
 /* Uploads a Kiddo-layout tree to the current HIP device and re-lays it out for the
  * kernels (synchronous). Replaces holding `&ImmutableKdTree` (kiddo src/lib.rs:270-271).

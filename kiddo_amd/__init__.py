@@ -95,6 +95,7 @@ SIGNATURES = {
     "kd_build_layout": ([_i32, _i32, _i32, _u64, _vp, _vp, _i32, C.POINTER(_vp)], C.c_int),
     "kd_layout_desc": ([_vp, C.POINTER(TreeDesc)], C.c_int),
     "kd_layout_free": ([_vp], None),
+    "kd_layout_from_donnelly": ([C.POINTER(TreeDesc), _i32, C.POINTER(_vp)], C.c_int),
     "kd_tree_create": ([C.POINTER(TreeDesc), _u32, C.POINTER(_vp)], C.c_int),
     "kd_tree_build_device": ([_i32, _i32, _i32, _u64, _vp, _vp, _u32, C.POINTER(_vp), _vp], C.c_int),
     "kd_tree_export_layout": ([_vp, C.POINTER(_vp)], C.c_int),
@@ -219,6 +220,18 @@ class HostLayout:
                 pass
             self._h = None
 
+    @classmethod
+    def from_donnelly(cls, dtype, K, bucket, size, max_stem_level, stems, extents, arena,
+                      block_height: int):
+        """Kiddo buffers of a Donnelly<block_height> tree -> the Eytzinger host layout
+        (kd_layout_from_donnelly)."""
+        d, keep = ImmutableKdTree._desc(dtype, K, bucket, size, max_stem_level, stems, extents, arena)
+        h = C.c_void_p()
+        _check(lib().kd_layout_from_donnelly(C.byref(d), block_height, C.byref(h)),
+               "kd_layout_from_donnelly")
+        del keep
+        return cls(_handle=h)
+
     @property
     def max_stem_level(self):
         return self.desc.max_stem_level
@@ -332,10 +345,8 @@ class ImmutableKdTree:
         _check(lib().kd_tree_export_layout(self._h, C.byref(h)), "kd_tree_export_layout")
         return HostLayout(_handle=h)
 
-    @classmethod
-    def from_layout(cls, dtype, K, bucket, size, max_stem_level, stems, extents, arena,
-                    shadow_f32: bool = False):
-        """Uploads a tree given in Kiddo's own buffers (stems, leaf extents, leaf arena)."""
+    @staticmethod
+    def _desc(dtype, K, bucket, size, max_stem_level, stems, extents, arena):
         stems = np.ascontiguousarray(stems)
         extents = np.ascontiguousarray(extents, dtype=np.uint64)
         arena = np.ascontiguousarray(arena, dtype=np.uint8)
@@ -351,7 +362,23 @@ class ImmutableKdTree:
         d.leaf_extents = extents.ctypes.data if extents.size else None
         d.leaf_bytes_len = arena.size
         d.leaf_bytes = arena.ctypes.data if arena.size else None
-        return cls(d, shadow_f32, keepalive=(stems, extents, arena))
+        return d, (stems, extents, arena)
+
+    @classmethod
+    def from_layout(cls, dtype, K, bucket, size, max_stem_level, stems, extents, arena,
+                    shadow_f32: bool = False):
+        """Uploads a tree given in Kiddo's own buffers (stems, leaf extents, leaf arena)."""
+        d, keep = cls._desc(dtype, K, bucket, size, max_stem_level, stems, extents, arena)
+        return cls(d, shadow_f32, keepalive=keep)
+
+    @classmethod
+    def from_donnelly(cls, dtype, K, bucket, size, max_stem_level, stems, extents, arena,
+                      block_height: int, shadow_f32: bool = False):
+        """Uploads a KdTree with Donnelly<block_height> stems (stems in Donnelly order, its
+        real leaves' extents, the arena), re-laid out by kd_layout_from_donnelly."""
+        lay = HostLayout.from_donnelly(dtype, K, bucket, size, max_stem_level, stems, extents, arena,
+                                       block_height)
+        return cls(lay.desc, shadow_f32, keepalive=lay)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -463,6 +463,144 @@ int kd_build_layout(int32_t dtype, int32_t dims, int32_t bucket, uint64_t n, con
   }
 }
 
+}  // extern "C"
+
+namespace kd {
+namespace {
+// A position in a Donnelly<bh> stem array (kiddo src/stem_strategy/donnelly/core.rs): the
+// array is cut into lines of 2^bh slots, each holding a triangle of bh levels (slot 0 of a
+// line's triangle at offset 0, the children of offset i at 2i + 1 and 2i + 2); the child of
+// a triangle's bottom row is the root of another line. `row` is the level inside the
+// triangle. The step is the reference's child-index rule (core.rs:254-289), in u32 with
+// wrap-around.
+struct DonnellyAt {
+  uint32_t slot, row;
+};
+DonnellyAt donnelly_child(uint32_t bh, DonnellyAt at, uint32_t right) {
+  const uint32_t in_line = at.slot & ((1u << bh) - 1u);
+  const uint32_t base = (at.slot & ~((1u << bh) - 1u)) + 1u;
+  if (at.row + 1u < bh) return {base + right + 2u * in_line, at.row + 1u};
+  const uint32_t column = in_line - at.row - 1u;
+  return {(base + 2u * column + right) << bh, 0u};
+}
+
+// Kiddo pads a block layout's stem depth to a multiple of the block height with levels
+// above the real root that hold no pivot and are always taken to the left
+// (construction.rs:431-444), so the real root sits `pad` levels down. In Eytzinger order
+// that is node 2^pad at level pad, and every real node keeps the level (hence split
+// dimension) it has in the Donnelly tree.
+template <class A> int from_donnelly_typed(const kd_tree_desc* d, uint32_t bh, HostLayout& L) {
+  const int D = d->max_stem_level + 1;
+  const uint64_t real = d->leaf_count;
+  if (real == 0 || (real & (real - 1)) != 0)
+    return set_error(KD_ERR_INVALID, "a Donnelly tree holds a power-of-two number of leaves, got %llu",
+                     (unsigned long long)real);
+  int dr = 0;
+  while ((uint64_t(1) << dr) < real) ++dr;
+  const int pad = D - dr;
+  if (D % int(bh) != 0 || pad < 0 || pad >= int(bh))
+    return set_error(KD_ERR_INVALID,
+                     "stem depth %d with %llu leaves is not a Donnelly<%u> layout (depth a multiple of %u, "
+                     "fewer than %u padding levels)", D, (unsigned long long)real, bh, bh, bh);
+  if (D > kMaxStemLevels) return set_error(KD_ERR_TOO_DEEP, "max_stem_level %d too deep", D - 1);
+  const uint64_t leaves = uint64_t(1) << D;
+  const uint64_t stems_len = ((uint64_t(2) << D) + 7) / 8 * 8;
+  const A* src = static_cast<const A*>(d->stems);
+  std::vector<A> eyt(stems_len, AxisHost<A>::max_value());
+  DonnellyAt root{0u, 0u};
+  for (int i = 0; i < pad; ++i) root = donnelly_child(bh, root, 0u);
+  struct Frame {
+    uint64_t node;
+    int level;
+    DonnellyAt at;
+  };
+  std::vector<Frame> stack{{uint64_t(1) << pad, pad, root}};
+  while (!stack.empty()) {
+    const Frame f = stack.back();
+    stack.pop_back();
+    if (f.at.slot >= d->stems_len)
+      return set_error(KD_ERR_INVALID, "stem slot %u of a level-%d node lies past the %llu stems", f.at.slot,
+                       f.level, (unsigned long long)d->stems_len);
+    eyt[f.node] = src[f.at.slot];
+    if (f.level + 1 < D) {
+      stack.push_back({2 * f.node, f.level + 1, donnelly_child(bh, f.at, 0u)});
+      stack.push_back({2 * f.node + 1, f.level + 1, donnelly_child(bh, f.at, 1u)});
+    }
+  }
+  L.dtype = d->dtype;
+  L.dims = d->dims;
+  L.bucket = d->bucket;
+  L.size = d->size;
+  L.max_stem_level = D - 1;
+  L.stems_len = stems_len;
+  L.stems.assign(reinterpret_cast<const uint8_t*>(eyt.data()),
+                 reinterpret_cast<const uint8_t*>(eyt.data()) + stems_len * sizeof(A));
+  // the arena, grown to a multiple of 8 bytes, where the empty leaves start (as an arena
+  // that appended them would place them, vec_of_arenas.rs:292-323)
+  const uint64_t end = (d->leaf_bytes_len + 7) / 8 * 8;
+  L.arena.assign(end, 0);
+  if (d->leaf_bytes_len) std::memcpy(L.arena.data(), d->leaf_bytes, d->leaf_bytes_len);
+  L.extents.assign(2 * leaves, 0);
+  L.max_leaf_len = 0;
+  for (uint64_t l = 0; l < leaves; ++l) {
+    const bool r = l < real;
+    L.extents[2 * l] = r ? d->leaf_extents[2 * l] : end;
+    L.extents[2 * l + 1] = r ? d->leaf_extents[2 * l + 1] : 0;
+    L.max_leaf_len = std::max<uint64_t>(L.max_leaf_len, L.extents[2 * l + 1]);
+  }
+  return KD_OK;
+}
+}  // namespace
+}  // namespace kd
+
+extern "C" {
+
+int kd_layout_from_donnelly(const kd_tree_desc* donnelly, int32_t block_height, kd_layout** out) {
+  if (!donnelly || !out) return kd::set_error(KD_ERR_INVALID, "null argument");
+  // the bottom row of a triangle starts at slot 2^(bh-1) - 1, which the child rule's
+  // column (slot - row - 1) only respects from bh = 3 up
+  if (block_height < 3 || block_height > 16)
+    return kd::set_error(KD_ERR_INVALID, "Donnelly block height %d outside 3..16", block_height);
+  if (donnelly->leaf_count && (!donnelly->leaf_extents || !donnelly->leaf_bytes))
+    return kd::set_error(KD_ERR_INVALID, "leaf buffers are null");
+  if (donnelly->stems_len && !donnelly->stems) return kd::set_error(KD_ERR_INVALID, "stems is null");
+  kd_layout* l = nullptr;
+  try {
+    l = new kd_layout;
+    int rc = KD_OK;
+    if (donnelly->max_stem_level < 0) {  // no stems: the layouts are the same
+      kd::HostLayout& L = l->L;
+      L.dtype = donnelly->dtype;
+      L.dims = donnelly->dims;
+      L.bucket = donnelly->bucket;
+      L.size = donnelly->size;
+      L.max_stem_level = -1;
+      L.stems_len = 0;
+      L.extents.assign(donnelly->leaf_extents, donnelly->leaf_extents + 2 * donnelly->leaf_count);
+      L.arena.assign(donnelly->leaf_bytes, donnelly->leaf_bytes + donnelly->leaf_bytes_len);
+      for (uint64_t i = 0; i < donnelly->leaf_count; ++i)
+        L.max_leaf_len = std::max<uint64_t>(L.max_leaf_len, L.extents[2 * i + 1]);
+    } else {
+      switch (donnelly->dtype) {
+        case KD_F32: rc = kd::from_donnelly_typed<float>(donnelly, uint32_t(block_height), l->L); break;
+        case KD_F64: rc = kd::from_donnelly_typed<double>(donnelly, uint32_t(block_height), l->L); break;
+        case KD_U16: rc = kd::from_donnelly_typed<uint16_t>(donnelly, uint32_t(block_height), l->L); break;
+        case KD_U32: rc = kd::from_donnelly_typed<uint32_t>(donnelly, uint32_t(block_height), l->L); break;
+        default: rc = kd::set_error(KD_ERR_UNSUPPORTED, "unsupported dtype %d", donnelly->dtype);
+      }
+    }
+    if (rc != KD_OK) {
+      delete l;
+      return rc;
+    }
+  } catch (const std::bad_alloc&) {
+    delete l;
+    return kd::set_error(KD_ERR_NO_MEMORY, "host allocation failed while re-laying out");
+  }
+  *out = l;
+  return KD_OK;
+}
+
 int kd_layout_desc(const kd_layout* layout, kd_tree_desc* out) {
   if (!layout || !out) return kd::set_error(KD_ERR_INVALID, "null argument");
   const kd::HostLayout& L = layout->L;

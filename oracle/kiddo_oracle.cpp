@@ -96,6 +96,7 @@ struct Layout {
   std::vector<uint64_t> extents;   // (byte offset, len) per leaf
   std::vector<uint8_t> arena;      // VecOfArenas leaf_bytes
   uint64_t max_leaf_len = 0;
+  int pad_levels = 0;              // sentinel levels above the real root (block layouts)
 };
 
 // ---------------------------------------------------------------- builder ---------
@@ -213,8 +214,14 @@ template <class A, int K> struct Builder {
   }
 };
 
+// block > 1: the stem depth is padded up to a multiple of `block` as Kiddo does for
+// block-based stem strategies (construction.rs:431-444): the padding levels sit above the
+// real root, hold no pivot (sentinel), and are pre-traversed to the left, so the real root
+// is Eytzinger node 2^pad at level pad and splits dimension pad % K; the soft leaf budget
+// is 2^(padded depth) (construction.rs:446-450). The leaves under the padding levels'
+// right children are empty and follow the real ones.
 template <class A, int K>
-int build_layout(Layout& L, const A* pts, const uint32_t* items, uint64_t n, int B) {
+int build_layout(Layout& L, const A* pts, const uint32_t* items, uint64_t n, int B, int block = 1) {
   L.K = K;
   L.B = B;
   L.n = n;
@@ -233,6 +240,9 @@ int build_layout(Layout& L, const A* pts, const uint32_t* items, uint64_t n, int
   } else {
     int d = 0;
     while ((uint64_t(1) << d) < leaf_node_count) ++d;  // log2(next_pow2)
+    const int pad = (block > 1 && d % block) ? block - d % block : 0;
+    d += pad;
+    if (d > 40) return -3;
     uint64_t budget = uint64_t(1) << d;
     uint64_t stem_count = uint64_t(1) << (d + 1);  // rightmost leaf's stem index + 1
     uint64_t padded = (stem_count + 7) / 8 * 8;     // construction.rs:465-469
@@ -241,8 +251,12 @@ int build_layout(Layout& L, const A* pts, const uint32_t* items, uint64_t n, int
     A* stems = reinterpret_cast<A*>(L.stems.data());
     for (uint64_t i = 0; i < padded; ++i) stems[i] = Ax<A>::maxv();
     L.max_stem_level = d - 1;
+    L.pad_levels = pad;
     Builder<A, K> b{pts, items, L, stems, d - 1, {}};
-    b.populate(1, 0, budget, idx.data(), n);
+    b.populate(uint64_t(1) << pad, pad, budget, idx.data(), n);
+    // the empty leaves under the padding levels' right children
+    const uint64_t real = uint64_t(1) << (d - pad);
+    for (uint64_t l = real; l < budget; ++l) append_leaf<A, K>(L, pts, items, idx.data(), 0);
   }
   if (L.max_leaf_len > (uint64_t)kMaxLeafLen) return -2;  // construction.rs:526
   return 0;
@@ -770,6 +784,113 @@ int or_build(int dtype, int K, int B, uint64_t n, const void* pts, const uint32_
   }
   *out = t;
   return 0;
+}
+
+// As or_build, with the stem depth padded to a multiple of `block` levels as for a
+// block-based stem strategy (Donnelly<block>); the tree is held in Eytzinger order.
+int or_build_blocked(int dtype, int K, int B, uint64_t n, const void* pts, const uint32_t* items,
+                     int block, or_tree** out) {
+  if (B < 1 || block < 1 || block > 16 || !out) return -1;
+  auto* t = new or_tree;
+  t->L.dtype = dtype;
+  int rc = 0;
+  DISPATCH(dtype, K, rc = (build_layout<A, KK>(t->L, static_cast<const A*>(pts), items, n, B, block)); break);
+  if (rc != 0) {
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return 0;
+}
+
+int or_pad_levels(const or_tree* t) { return t->L.pad_levels; }
+
+// Donnelly<BH> stem order (src/stem_strategy/donnelly/core.rs): stems are laid out in
+// minor triangles of BH levels, one per line of 2^BH slots. One step down
+// (step_pure, core.rs:254-289): within a triangle the children of slot i are 2i + 1 and
+// 2i + 2 of the line (offset by the line base); from the bottom row of a triangle (minor
+// level BH - 1) the child is the root of the child triangle, whose line starts at
+// (line base + 1 + 2 * column + is_right) << BH, column = slot - minor level - 1.
+// All arithmetic wraps in u32, as in the reference.
+struct DonPos {
+  uint32_t idx, minor;
+};
+static DonPos donnelly_step(uint32_t BH, DonPos p, bool right) {
+  const uint32_t r = right ? 1u : 0u;
+  const uint32_t mask = (1u << BH) - 1u;
+  const uint32_t min_idx = p.idx & mask;
+  const uint32_t col = min_idx - p.minor - 1u;
+  const uint32_t base = (p.idx & ~mask) + 1u;
+  const uint32_t same = base + r + (min_idx << 1);
+  const uint32_t next = ((base + (col << 1)) << BH) + (r << BH);
+  const bool major = p.minor + 1u == BH;
+  return DonPos{major ? next : same, major ? 0u : p.minor + 1u};
+}
+// traverse_block (core.rs:161-200): from a triangle root (minor level 0), `block_size`
+// levels at once to child `child` (0 .. 2^block_size - 1) of the block.
+static uint32_t donnelly_block_step(uint32_t BH, uint32_t idx, uint32_t child, uint32_t block_size) {
+  const uint32_t mask = (1u << BH) - 1u;
+  const uint32_t off = (1u << BH) - (block_size << 1) - 1u;
+  return ((idx & ~mask) + off + child) << BH;
+}
+
+// Stem index (and minor level) reached by following `n` choices from the root.
+int or_donnelly_path(int BH, const uint8_t* bits, uint64_t n, uint32_t* out2) {
+  if (BH < 1 || BH > 16) return -1;
+  DonPos p{0, 0};
+  for (uint64_t i = 0; i < n; ++i) p = donnelly_step(uint32_t(BH), p, bits[i] != 0);
+  out2[0] = p.idx;
+  out2[1] = p.minor;
+  return 0;
+}
+// Stem index reached by `n` block steps of `block_size` levels (children in [0, 2^bs)).
+int or_donnelly_block_path(int BH, const uint8_t* children, uint64_t n, int block_size, uint32_t* out) {
+  if (BH < 1 || BH > 16 || block_size < 1) return -1;
+  uint32_t idx = 0;
+  for (uint64_t i = 0; i < n; ++i) idx = donnelly_block_step(uint32_t(BH), idx, children[i], uint32_t(block_size));
+  *out = idx;
+  return 0;
+}
+
+// The stems of a padded tree (or_build_blocked with block = BH) in Donnelly<BH> order,
+// sized as Kiddo sizes them (construction.rs:441-469: from the real root's position,
+// follow the right-most path of stem-depth levels; that index + 1, rounded up to a
+// multiple of 8), unused slots holding the sentinel. out == null: only *out_len is set.
+int or_donnelly_stems(const or_tree* t, int BH, void* out, uint64_t* out_len) {
+  const Layout& L = t->L;
+  if (L.max_stem_level < 0) {
+    *out_len = 0;
+    return 0;
+  }
+  const int D = L.max_stem_level + 1, pad = L.pad_levels;
+  if (BH < 3 || BH > 16 || D % BH != 0 || pad >= BH) return -2;  // BH < 3: the child rule wraps
+  DonPos root{0, 0};
+  for (int i = 0; i < pad; ++i) root = donnelly_step(uint32_t(BH), root, false);
+  DonPos far = root;
+  for (int i = 0; i < D; ++i) far = donnelly_step(uint32_t(BH), far, true);
+  const uint64_t len = (uint64_t(far.idx) + 1 + 7) / 8 * 8;
+  *out_len = len;
+  if (!out) return 0;
+  DISPATCH(L.dtype, L.K, {
+    (void)KK;
+    A* dst = static_cast<A*>(out);
+    const A* src = reinterpret_cast<const A*>(L.stems.data());
+    for (uint64_t i = 0; i < len; ++i) dst[i] = Ax<A>::maxv();
+    // depth-first over the real subtree: Eytzinger node e at level l <-> Donnelly slot
+    struct Frame { uint64_t e; int level; DonPos p; };
+    std::vector<Frame> stack{{uint64_t(1) << pad, pad, root}};
+    while (!stack.empty()) {
+      const Frame f = stack.back();
+      stack.pop_back();
+      if (f.p.idx >= len) return -3;
+      dst[f.p.idx] = src[f.e];
+      if (f.level + 1 < D) {
+        stack.push_back({2 * f.e, f.level + 1, donnelly_step(uint32_t(BH), f.p, false)});
+        stack.push_back({2 * f.e + 1, f.level + 1, donnelly_step(uint32_t(BH), f.p, true)});
+      }
+    }
+    return 0;
+  });
 }
 
 // Wraps an existing Kiddo-layout tree (for example one made by another builder).

@@ -44,6 +44,11 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, u64, i32, f64 = C.c_void_p, C.c_uint64, C.c_int, C.c_double
         L.or_build.argtypes = [i32, i32, i32, u64, vp, vp, C.POINTER(vp)]
+        L.or_build_blocked.argtypes = [i32, i32, i32, u64, vp, vp, i32, C.POINTER(vp)]
+        L.or_pad_levels.argtypes = [vp]
+        L.or_donnelly_path.argtypes = [i32, vp, u64, vp]
+        L.or_donnelly_block_path.argtypes = [i32, vp, u64, i32, vp]
+        L.or_donnelly_stems.argtypes = [vp, i32, vp, C.POINTER(u64)]
         L.or_from_layout.argtypes = [i32, i32, i32, u64, i32, vp, u64, vp, u64, vp, u64,
                                      C.POINTER(vp)]
         L.or_free.argtypes = [vp]
@@ -104,6 +109,25 @@ def metric_rect(flags: int, rd: float, off, dim: int, new_off: float) -> float:
     return float(out[0])
 
 
+def donnelly_path(block_height: int, path) -> tuple:
+    """(stem index, minor level) reached from the root of a Donnelly<block_height> stem
+    array by the left (False) / right (True) choices of `path` (donnelly/core.rs
+    traverse)."""
+    bits = np.ascontiguousarray([1 if b else 0 for b in path] or [0], np.uint8)
+    out = np.zeros(2, np.uint32)
+    _check(lib().or_donnelly_path(block_height, _p(bits), len(path), _p(out)), "donnelly_path")
+    return int(out[0]), int(out[1])
+
+
+def donnelly_block_path(block_height: int, children, block_size: int) -> int:
+    """Stem index reached by block steps of `block_size` levels (traverse_block)."""
+    ch = np.ascontiguousarray(list(children) or [0], np.uint8)
+    out = np.zeros(1, np.uint32)
+    _check(lib().or_donnelly_block_path(block_height, _p(ch), len(children), block_size, _p(out)),
+           "donnelly_block_path")
+    return int(out[0])
+
+
 def update_pivot(points: np.ndarray, sort_index: np.ndarray, dim: int, init: int) -> int:
     """construction.rs:1089-1175 on one slice; sort_index is reordered in place."""
     pts = np.ascontiguousarray(points)
@@ -127,15 +151,40 @@ class Tree:
 
     # -- construction -------------------------------------------------------------
     @classmethod
-    def build(cls, points: np.ndarray, bucket: int = 32, items: np.ndarray | None = None):
-        """ImmutableKdTree::new_from_slice (items None) / new_from_entries (items given)."""
+    def build(cls, points: np.ndarray, bucket: int = 32, items: np.ndarray | None = None,
+              block: int | None = None):
+        """ImmutableKdTree::new_from_slice (items None) / new_from_entries (items given).
+        block: the stem depth padded to a multiple of `block` levels, as Kiddo builds a tree
+        with a block-based stem strategy (Donnelly<block>); held in Eytzinger order."""
         pts = np.ascontiguousarray(points)
         code = DTYPES[pts.dtype]
         n, K = pts.shape
         it = None if items is None else np.ascontiguousarray(items, dtype=np.uint32)
         h = C.c_void_p()
-        _check(lib().or_build(code, K, bucket, n, _p(pts), _p(it), C.byref(h)), "build")
+        if block is None:
+            _check(lib().or_build(code, K, bucket, n, _p(pts), _p(it), C.byref(h)), "build")
+        else:
+            _check(lib().or_build_blocked(code, K, bucket, n, _p(pts), _p(it), block, C.byref(h)),
+                   "build_blocked")
         return cls(h, code, K, bucket)
+
+    @property
+    def pad_levels(self) -> int:
+        return int(lib().or_pad_levels(self._h))
+
+    def donnelly_layout(self, block_height: int):
+        """The tree as Kiddo holds a KdTree<_, _, Donnelly<block_height>, ...>: stems in
+        Donnelly order, the real leaves' extents (the empty ones under the padding levels
+        dropped), the arena and max_stem_level. Needs a tree built with block=block_height."""
+        n = C.c_uint64()
+        _check(lib().or_donnelly_stems(self._h, block_height, None, C.byref(n)), "donnelly_stems")
+        stems = np.zeros(n.value, A_DTYPE[self.dtype_code])
+        if n.value:
+            _check(lib().or_donnelly_stems(self._h, block_height, _p(stems), C.byref(n)), "donnelly_stems")
+        ext = self.extents()
+        if self.max_stem_level >= 0:
+            ext = ext[: 1 << (self.max_stem_level + 1 - self.pad_levels)]
+        return stems, np.ascontiguousarray(ext), self.arena(), self.max_stem_level
 
     @classmethod
     def from_layout(cls, dtype, K, B, n, max_stem_level, stems, extents, arena):
