@@ -140,8 +140,8 @@ template <class A, int K> int create_typed(const kd_tree_desc* desc, uint32_t fl
   if (msl >= 0 && desc->stems_len < (uint64_t(1) << (msl + 1)))
     return set_error(KD_ERR_INVALID, "stems_len %llu too short", (unsigned long long)desc->stems_len);
   if (n > 0xFFFFFFFFull) return set_error(KD_ERR_INVALID, "trees above 2^32-1 entries are not supported");
-  if ((flags & KD_TREE_SHADOW_F32) && (desc->dtype != KD_F64 || K > 3))
-    return set_error(KD_ERR_UNSUPPORTED, "the FP32 shadow is provided for f64 trees with dims <= 3");
+  if ((flags & KD_TREE_SHADOW_F32) && desc->dtype != KD_F64)
+    return set_error(KD_ERR_UNSUPPORTED, "the FP32 shadow is provided for f64 trees");
   const size_t entry_bytes = K * sizeof(A) + 4;
   std::vector<uint32_t> leaf_start(L + 1, 0);
   uint64_t total = 0, max_len = 0;
@@ -214,11 +214,15 @@ template <class A, int K> int create_typed(const kd_tree_desc* desc, uint32_t fl
         return set_error(KD_ERR_UNSUPPORTED, "FP32 shadow needs |coordinates| <= 2^60");
     host_parallel(n, [&](uint64_t s, uint64_t e) {
       for (uint64_t i = s; i < e; ++i) {
+        // {x, y, z, item bits} for K <= 3, {x, y, z, w} for K = 4
         float4 f;
         f.x = float(coords[0 * plane + i]);
         f.y = float(coords[1 * plane + i]);
         f.z = K > 2 ? float(coords[2 * plane + i]) : 0.0f;
-        std::memcpy(&f.w, &items[i], 4);
+        if (K > 3)
+          f.w = float(coords[3 * plane + i]);
+        else
+          std::memcpy(&f.w, &items[i], 4);
         shadow[i] = f;
       }
     });

@@ -676,20 +676,40 @@ template <class O, int MAXK> struct BestNVis {
 //   |fl32(p) - p| <= u|p| + eta, so each FP32 coordinate difference delta_i satisfies
 //   |delta_i - (p_i - q_i)| <= u(2+u)(|p_i| + |q_i|) + 3 eta <= eps (per query, from the
 //   largest |coordinate| of the tree and the query), hence ||delta|| <= ||t|| + sqrt(K) eps.
-//   The FP32 sum of squares of non-negative terms is at most ||delta||^2 (1+u)^K, and the
-//   exact distance is at most the FP64 one times (1 + 1e-15). So FP64 dist <= T implies
-//   FP32 dist <= (sqrt(T (1+1e-15)) + sqrt(K) eps)^2 (1 + 4u) + 1e-44, rounded up.
+//   The FP32 sum of K <= 4 non-negative squares is at most ||delta||^2 (1+u)^K <=
+//   ||delta||^2 (1 + 5u), plus K 2^-150 < 1e-44 when products fall in the subnormal range,
+//   and the exact distance is at most the FP64 one times (1 + 1e-15). So FP64 dist <= T
+//   implies FP32 dist <= (sqrt(T (1+1e-15)) + sqrt(K) eps)^2 (1 + 5u) + 1e-44, rounded up.
 // The square is expanded as T' + 2 e s + e^2 with s >= sqrt(T'): an FP32 square root of T'
 // rounded up to FP32, inflated by 2^-19 (far above any FP32 square root's error), so the
 // threshold costs one FP32 square root instead of an FP64 one.
 __device__ __forceinline__ float next_up(float f) { return __uint_as_float(__float_as_uint(f) + 1u); }
+
+// FP32 squared distance of a shadow entry in Kiddo's order, every product and sum rounded.
+// The shadow holds {x, y, z, item bits} for K <= 3 and {x, y, z, w} for K = 4.
+template <int K>
+__device__ __forceinline__ float shadow_sq(const float4 sh, const float (&qf)[4]) {
+  const float dx = sub_rn(sh.x, qf[0]);
+  float acc = mul_rn(dx, dx);
+  const float dy = sub_rn(sh.y, qf[1]);
+  acc = add_rn(acc, mul_rn(dy, dy));
+  if (K > 2) {
+    const float dz = sub_rn(sh.z, qf[2]);
+    acc = add_rn(acc, mul_rn(dz, dz));
+  }
+  if (K > 3) {
+    const float dw = sub_rn(sh.w, qf[3]);
+    acc = add_rn(acc, mul_rn(dw, dw));
+  }
+  return acc;
+}
 __device__ __forceinline__ float screen_threshold(double T, double eps_k) {
   if (!(T < __builtin_huge_val())) return __builtin_huge_valf();
   const double tp = T * (1.0 + 1e-15);
   float tf = (float)tp;
   if ((double)tf < tp) tf = next_up(tf);  // tf >= tp (tp >= 0)
   const double s = double(sqrtf(tf)) * (1.0 + 0x1p-19);
-  const double tt = (tp + 2.0 * eps_k * s + eps_k * eps_k) * (1.0 + 4.0 * 0x1p-24) + 1e-44;
+  const double tt = (tp + 2.0 * eps_k * s + eps_k * eps_k) * (1.0 + 5.0 * 0x1p-24) + 1e-44;
   float f = (float)tt;
   if ((double)f < tt) f = next_up(f);
   return f;
@@ -699,10 +719,10 @@ __device__ __forceinline__ float screen_threshold(double T, double eps_k) {
 // the error of an FP32 coordinate difference (see screen_threshold).
 template <int K>
 __device__ __forceinline__ void screen_setup(const Small<double, K>& q, const double* __restrict__ max_abs,
-                                             float (&qf)[3], double& eps_k) {
+                                             float (&qf)[4], double& eps_k) {
   double amax = 0.0;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) qf[d] = 0.0f;
+  for (int d = 0; d < 4; ++d) qf[d] = 0.0f;
 #pragma unroll
   for (int d = 0; d < K; ++d) {
     qf[d] = (float)q[d];
@@ -818,7 +838,7 @@ template <class O, bool SCREEN> struct HeapKnnVis {
   RowHeap<O, KnnKey> h;
   O radius;
   bool excl;
-  float qf[3];   // SCREEN: the query in FP32
+  float qf[4];   // SCREEN: the query in FP32
   double eps_k;  // SCREEN: sqrt(K) * per-query coordinate-difference error bound
   float thr;     // SCREEN: FP32 threshold of the current bound
   uint32_t rescored;
@@ -832,14 +852,7 @@ template <class O, bool SCREEN> struct HeapKnnVis {
     for (uint32_t p = s; p < e; ++p) {
       if constexpr (SCREEN) {
         const float4 sh = t.shadow[p];
-        const float dx = sub_rn(sh.x, qf[0]);
-        float acc = mul_rn(dx, dx);
-        const float dy = sub_rn(sh.y, qf[1]);
-        acc = add_rn(acc, mul_rn(dy, dy));
-        if (K > 2) {
-          const float dz = sub_rn(sh.z, qf[2]);
-          acc = add_rn(acc, mul_rn(dz, dz));
-        }
+        const float acc = shadow_sq<K>(sh, qf);
         if (acc > thr) continue;
         ++rescored;
       }
@@ -927,7 +940,7 @@ template <int W> struct EntryMask {
 
 template <int MAXK> struct ScreenKnnVis {
   KnnVis<double, MAXK> v;
-  float qf[3];
+  float qf[4];
   double eps_k;
   float thr;
   uint32_t rescored;
@@ -940,14 +953,7 @@ template <int MAXK> struct ScreenKnnVis {
   template <int K>
   __device__ __forceinline__ float screen_dist(const DevTree<double, K>& t, uint32_t p) const {
     const float4 sh = t.shadow[p];
-    const float dx = sub_rn(sh.x, qf[0]);
-    float acc = mul_rn(dx, dx);
-    const float dy = sub_rn(sh.y, qf[1]);
-    acc = add_rn(acc, mul_rn(dy, dy));
-    if (K > 2) {
-      const float dz = sub_rn(sh.z, qf[2]);
-      acc = add_rn(acc, mul_rn(dz, dz));
-    }
+    const float acc = shadow_sq<K>(sh, qf);
     return acc;
   }
   // The next survivor at or after entry `next` past the mask (entries screened one at a
@@ -990,14 +996,7 @@ template <int MAXK> struct ScreenKnnVis {
         uint32_t bits = 0;
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u) {
-          const float dx = sub_rn(sh[u].x, qf[0]);
-          float acc = mul_rn(dx, dx);
-          const float dy = sub_rn(sh[u].y, qf[1]);
-          acc = add_rn(acc, mul_rn(dy, dy));
-          if (K > 2) {
-            const float dz = sub_rn(sh[u].z, qf[2]);
-            acc = add_rn(acc, mul_rn(dz, dz));
-          }
+          const float acc = shadow_sq<K>(sh[u], qf);
           bits |= ((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << u;
         }
         word |= uint64_t(bits) << (j0 - lo);
@@ -1344,7 +1343,7 @@ template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {
   bool excl;
   O kd;          // k-th (last kept) entry once full, group-uniform
   uint32_t kit;
-  float qf[3];   // SCREEN: the query in FP32
+  float qf[4];   // SCREEN: the query in FP32
   double eps_k;  // SCREEN: sqrt(K) * per-query coordinate-difference error bound
   float thr;     // SCREEN: FP32 threshold certified to admit every FP64 candidate
   uint32_t rescored;
@@ -1404,18 +1403,11 @@ template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {
       if (p < e) {
         if constexpr (SCREEN) {
           const float4 sh = t.shadow[p];
-          const float dx = sub_rn(sh.x, qf[0]);
-          float acc = mul_rn(dx, dx);
-          const float dy = sub_rn(sh.y, qf[1]);
-          acc = add_rn(acc, mul_rn(dy, dy));
-          if (K > 2) {
-            const float dz = sub_rn(sh.z, qf[2]);
-            acc = add_rn(acc, mul_rn(dz, dz));
-          }
+          const float acc = shadow_sq<K>(sh, qf);
           if (!(acc > thr)) {
             ++rescored;
             d = point_dist(t, p, qw);
-            it = __float_as_uint(sh.w);
+            it = K <= 3 ? __float_as_uint(sh.w) : t.items[p];  // K = 4: w is a coordinate
             ok = true;
           }
         } else {

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_best_n_within(DevTree<A, K> t, const
 
 // nearest_n / nearest_n().within for k above the register forms (k > 64): the top-k is a
 // max-heap in the query's output row (device.h RowHeap), sorted in place at the end.
-// SCREEN: f64 trees with K <= 3 and the FP32 shadow (SquaredEuclidean only).
+// SCREEN: f64 trees with the FP32 shadow (SquaredEuclidean only).
 template <class A, int K, class M, bool SCREEN>
 __global__ __launch_bounds__(kBlock) void k_nearest_n_heap(DevTree<A, K> t, const A* __restrict__ queries,
                                                            const uint32_t* __restrict__ order, uint64_t m, M met,
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_nearest_n_heap(DevTree<A, K> t, cons
     v.rescored = 0;
     v.thr = __builtin_huge_valf();
     if constexpr (SCREEN) {
-      static_assert(std::is_same<A, double>::value && K <= 3, "the FP32 screen is for f64 trees, K <= 3");
+      static_assert(std::is_same<A, double>::value, "the FP32 screen is for f64 trees");
       screen_setup<K>(q, max_abs, v.qf, v.eps_k);
       v.thr = screen_threshold(double(v.bound()), v.eps_k);
     }

@@ -202,7 +202,10 @@ __global__ void k_gather_tree(const A* __restrict__ pts, const uint32_t* __restr
   }
   const uint32_t it = items_in ? items_in[p] : p;
   items[i] = it;
-  if (shadow) shadow[i] = make_float4(float(c[0]), float(c[1]), K > 2 ? float(c[2]) : 0.0f, __uint_as_float(it));
+  // {x, y, z, item bits} for K <= 3, {x, y, z, w} for K = 4
+  if (shadow)
+    shadow[i] = make_float4(float(c[0]), float(c[1]), K > 2 ? float(c[2]) : 0.0f,
+                            K > 3 ? float(c[3]) : __uint_as_float(it));
 }
 
 }  // namespace dev
@@ -413,8 +416,8 @@ int build_device(kd_tree* t, int dtype, int dims, int bucket, uint64_t n, const 
   if (bucket < 1) return set_error(KD_ERR_INVALID, "bucket size must be >= 1");
   if (n > 0 && !points) return set_error(KD_ERR_INVALID, "points is null");
   if (n >= 0x7FFFFFFFull) return set_error(KD_ERR_UNSUPPORTED, "device builds hold < 2^31 entries");
-  if ((flags & KD_TREE_SHADOW_F32) && (dtype != KD_F64 || dims > 3))
-    return set_error(KD_ERR_UNSUPPORTED, "the FP32 shadow is provided for f64 trees with dims <= 3");
+  if ((flags & KD_TREE_SHADOW_F32) && dtype != KD_F64)
+    return set_error(KD_ERR_UNSUPPORTED, "the FP32 shadow is provided for f64 trees");
   t->dtype = dtype;
   t->dims = dims;
   t->bucket = bucket;

@@ -17,15 +17,16 @@ namespace dev {
 // (the smallest block holding 4k entries, 64 at least for k >= 8; leaves under one stem
 // are contiguous) give a k-th smallest S32, and every one of those k entries has an FP64
 // distance of at most
-// (sqrt((S32 + 1e-44) (1+4u)) + sqrt(K) eps)^2 (1 + 2e-15): the FP32 sum of non-negative
-// squares is at least ||delta||^2 (1-u)^K - K 2^-150 (a product that underflows into the
-// subnormal range loses up to 2^-150 absolute, a sum with a subnormal result is exact;
-// 1e-44 > 3 * 2^-150), each FP32 coordinate difference is within eps of the exact one,
+// (sqrt((S32 + 1e-44) (1+5u)) + sqrt(K) eps)^2 (1 + 2e-15): the FP32 sum of K <= 4
+// non-negative squares is at least ||delta||^2 (1-u)^K - K 2^-150 (a product that
+// underflows into the subnormal range loses up to 2^-150 absolute, a sum with a subnormal
+// result is exact; 1e-44 > 4 * 2^-150, and 1 / (1-u)^4 < 1 + 5u), each FP32 coordinate
+// difference is within eps of the exact one,
 // and the FP64 evaluation is within (1 + 1e-15) of the exact square. Returns
 // +inf when the tree holds fewer than k entries. `up` receives the block's height in
 // levels (2^up leaves) and [b, e) its entries; 0 and an empty range with an infinite cap.
 template <int K, int MAXK>
-__device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K>& q, const float (&qf)[3],
+__device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K>& q, const float (&qf)[4],
                               double eps_k, int k, int& up_out, uint32_t& b_out, uint32_t& e_out) {
   up_out = 0;
   b_out = e_out = 0;
@@ -65,14 +66,7 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
 #pragma unroll
     for (uint32_t u = 0; u < kBatch; ++u) {
       const float4 sh = t.shadow[p0 + u];  // up to 7 past e: the shadow's padding
-      const float dx = sub_rn(sh.x, qf[0]);
-      float acc = mul_rn(dx, dx);
-      const float dy = sub_rn(sh.y, qf[1]);
-      acc = add_rn(acc, mul_rn(dy, dy));
-      if (K > 2) {
-        const float dz = sub_rn(sh.z, qf[2]);
-        acc = add_rn(acc, mul_rn(dz, dz));
-      }
+      const float acc = shadow_sq<K>(sh, qf);
       d32[u] = p0 + u < e ? acc : __builtin_huge_valf();
     }
 #pragma unroll
@@ -90,7 +84,7 @@ __device__ double prescan_cap(const DevTree<double, K>& t, const Small<double, K
   up_out = up;
   b_out = b;
   e_out = e;
-  const double r = sqrt((double(kth) + 1e-44) * (1.0 + 4.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
+  const double r = sqrt((double(kth) + 1e-44) * (1.0 + 5.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
   return r * r * (1.0 + 2e-15);
 }
 
@@ -156,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_group(DevTree<A, K> t, const A* 
     GroupKnnVis<A, K, G, SCREEN> v;
     v.init(k, radius, excl != 0, gl);
     if constexpr (SCREEN) {
-      if constexpr (std::is_same<A, double>::value && K <= 3) {
+      if constexpr (std::is_same<A, double>::value) {
         screen_setup<K>(q, max_abs, v.qf, v.eps_k);
         v.thr = screen_threshold(double(v.bound()), v.eps_k);
       }
@@ -201,7 +195,7 @@ int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, v
   const size_t lds = size_t(dt.lds_n) * sizeof(A);
   reset_counters(t, a.s);
   KernelTimer kt(t, a.s);
-  if constexpr (std::is_same<A, double>::value && K <= 3) {
+  if constexpr (std::is_same<A, double>::value) {
     if (a.flags & KD_SCREEN_F32) {
       hipLaunchKernelGGL((dev::k_nearest_n_screen<K, MAXK, WAVES>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds, a.s,
                          dt, q, a.order, a.m, k, radius, excl, t->d_max_abs, items,
@@ -229,7 +223,7 @@ int launch_group(const QueryArgs& a, int k, double radius, uint32_t* items, void
   reset_counters(t, a.s);
   KernelTimer kt(t, a.s);
   if (a.flags & KD_SCREEN_F32) {
-    if constexpr (std::is_same<A, double>::value && K <= 3) {
+    if constexpr (std::is_same<A, double>::value) {
       hipLaunchKernelGGL((dev::k_knn_group<A, K, G, true>), dim3(grid), dim3(dev::kBlock), lds, a.s, dt, q,
                          a.order, a.m, k, O(radius), excl, t->d_max_abs, items, static_cast<O*>(dists), counts,
                          t->counters);
@@ -247,14 +241,14 @@ template <class A, int K>
 int launch_typed(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts) {
   kd_tree* t = a.t;
   if (a.flags & KD_SCREEN_F32) {
-    if (!(std::is_same<A, double>::value && K <= 3))
-      return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 applies to f64 trees with dims <= 3");
+    if (!(std::is_same<A, double>::value))
+      return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 applies to f64 trees");
     if (!t->shadow)
       return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
   }
   // above the register forms: the top-k in a heap in each query's output row
   if (k > kMaxK) {
-    if constexpr (std::is_same<A, double>::value && K <= 3)
+    if constexpr (std::is_same<A, double>::value)
       if (a.flags & KD_SCREEN_F32) return mlaunch_knn_heap<A, K, dev::SqEuclidean, true>(a, {}, k, radius, items, dists, counts);
     return mlaunch_knn_heap<A, K, dev::SqEuclidean, false>(a, {}, k, radius, items, dists, counts);
   }

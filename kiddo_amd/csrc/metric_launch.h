@@ -270,7 +270,7 @@ int mlaunch_knn_typed(const QueryArgs& a, M met, int k, double radius, uint32_t*
 }
 
 // nearest_n for k above the register forms (any metric; SCREEN: SquaredEuclidean f64 trees
-// with K <= 3 and the FP32 shadow)
+// with the FP32 shadow)
 template <class A, int K, class M, bool SCREEN>
 int mlaunch_knn_heap(const QueryArgs& a, M met, int k, double radius, uint32_t* items, void* dists,
                      uint32_t* counts) {

@@ -4,7 +4,8 @@
     (bucket 64 and 128, long runs of equal points): with 64 lanes per query the hits of
     lanes 32..63 must be offered as well;
   * the FP32 screen on a tree whose coordinates are all about 1e-22 (FP32 squares in the
-    subnormal range: the certified bounds must keep their additive slack);
+    subnormal range: the certified bounds must keep their additive slack), and on f64
+    trees with K = 4 (thread, group and heap forms);
   * NaN queries (the reference compares NaN distances as false everywhere: nothing is
     ever accepted, a NaN query returns no results);
   * the reference's CPU-runnable case at its own size: 100,000 uniform f64 points,
@@ -144,3 +145,43 @@ def test_reference_cpu_case_full_size(ka, oracle_mod, cuda):
         r = t2.nearest_n_raw(q, 8, flags=flags | ka.KD_SCREEN_F32)
         _cmp((r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy()), ref,
              "screened nearest_n(8)")
+
+
+@pytest.mark.parametrize("kind", ["uniform", "grid", "tiny"])
+def test_screen_four_dims(ka, oracle_mod, cuda, kind):
+    """The FP32 screen on f64 trees with K = 4 (the shadow holds the fourth coordinate and
+    the items come from the item array): thread, group and heap (k > 64) forms, with and
+    without a radius, against the oracle."""
+    rng = np.random.default_rng(29)
+    if kind == "uniform":
+        pts = rng.random((60_000, 4))
+    elif kind == "grid":
+        pts = rng.integers(0, 6, (20_000, 4)).astype(np.float64)
+    else:
+        pts = rng.random((8_000, 4)) * 1e-22
+    q = rng.random((2500, 4)) * (float(pts.max()) + (1.0 if kind == "grid" else 0.0))
+    q[:400] = pts[rng.integers(0, len(pts), 400)]
+    t = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
+    O = OracleEngine(oracle_mod)
+    ot = O.build(pts, 32)
+    r2 = float(np.median(O.nearest_n(ot, q[:200], 8)[1][:, -1]))
+    for k, radius, excl in ((1, math.inf, False), (8, math.inf, False), (16, math.inf, False),
+                            (16, r2, False), (16, r2, True), (100, math.inf, False)):
+        ref = O.nearest_n(ot, q, k, radius, excl)
+        for variant in (1, 2):
+            t.tune(variant, 0, -1)
+            for flags in (ka.KD_SCREEN_F32, ka.KD_SCREEN_F32 | ka.KD_MORTON):
+                f = flags | (ka.KD_EXCLUSIVE if excl else 0)
+                _announce("screen K=4", kind, k, radius, excl, variant, f)
+                r = t.nearest_n_raw(q, k, radius, f)
+                _cmp((r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy()), ref,
+                     f"{kind} k={k} r={radius} excl={excl} variant={variant} flags={f}")
+    t.tune(0, 0, -1)
+    t.nearest_n_raw(q, 16, flags=ka.KD_SCREEN_F32)
+    st = t.query_stats()
+    assert 0 < st["rescored"] <= st["points"], st
+    # built on the GPU: the same shadow layout
+    gt = ka.ImmutableKdTree.build_on_device(pts, 32, shadow_f32=True)
+    r = gt.nearest_n_raw(q, 16, flags=ka.KD_SCREEN_F32 | ka.KD_MORTON)
+    _cmp((r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy()), O.nearest_n(ot, q, 16),
+         f"{kind} device-built screened k=16")
