@@ -78,6 +78,12 @@ def main():
     ap.add_argument("--result-budget", type=int, default=8 << 30,
                     help="mixture: bytes of within results held at once (queries run in chunks)")
     ap.add_argument("--rate", type=float, default=4000.0, help="serving: Poisson arrivals per second")
+    ap.add_argument("--morton-min", type=int, default=4096,
+                    help="serving: requests of at least this many queries are Morton-ordered")
+    ap.add_argument("--max-part", type=int, default=16384,
+                    help="serving: requests are cut into parts of at most this many queries (0: never)")
+    ap.add_argument("--rates", default="2000,8000,16000,32000",
+                    help="serving: arrival rates (per second) of the rate sweep before the main run")
     ap.add_argument("--trees", type=int, default=4096)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
@@ -493,40 +499,61 @@ def bench_serving(args, torch, ka):
         else:
             q = palette_u16(m, seed=200_000 + i)
             radius = float(4 * 2000 ** 2)
-        flags = ka.KD_MORTON if m >= 4096 else 0
+        flags = ka.KD_MORTON if m >= args.morton_min else 0
         reqs.append((tree, fam, q, 5, radius if fam == ka.FAMILY_BEST_N_WITHIN else np.inf, flags))
-    gaps = rng.exponential(1.0 / args.rate, args.requests)
-    arrivals = np.cumsum(gaps)
     total_q = sum(r[2].shape[0] for r in reqs)
-    log(f"{args.requests} requests, {total_q} queries, offered {total_q / arrivals[-1] / 1e6:.1f} Mq/s")
-    srv = ka.Server([sphere, palette], arena_bytes=1 << 30, slots=4, max_coalesce=1 << 18)
-    # warm-up: one request of each family and tree
-    for tree in (0, 1):
-        for fam in fams:
-            r = reqs[0]
-            q = latlon_clusters(256, seed=7) if tree == 0 else palette_u16(256, seed=7)
-            srv.wait(srv.submit(tree, fam, q, 5, r[4]))
-    tickets = [None] * len(reqs)
-    lat = [None] * len(reqs)
-    start = time.perf_counter()
+    log(f"{args.requests} requests, {total_q} queries")
+    srv = ka.Server([sphere, palette], arena_bytes=1 << 30, slots=4, max_coalesce=1 << 16,
+                    max_part=args.max_part)
+    # warm-up: a sample of the same request mix (every family on both trees, small and large,
+    # with and without Morton ordering), so that one-time set-up (first launches of each
+    # kernel, the sort's memory pool) happens before the timed run
+    warm = [reqs[i] for i in np.random.default_rng(args.seed + 9).choice(len(reqs), 60, replace=False)]
+    for w in range(2):
+        for r in [srv.submit(*x) for x in warm]:
+            srv.wait(r)
 
-    def producer():
-        for i, (tree, fam, q, k, radius, flags) in enumerate(reqs):
-            while time.perf_counter() - start < arrivals[i]:
+    def run(rate, count):
+        """Poisson arrivals at `rate` per second of the first `count` requests; returns the
+        latencies (ms), the elapsed seconds and the arrival times."""
+        sel = reqs[:count]
+        arr = np.cumsum(np.random.default_rng(args.seed + 4 + int(rate)).exponential(1.0 / rate, count))
+        tickets = [None] * count
+        lat = [None] * count
+        start = time.perf_counter()
+
+        def producer():
+            for i, (tree, fam, q, k, radius, flags) in enumerate(sel):
+                while time.perf_counter() - start < arr[i]:
+                    time.sleep(0.0001)
+                tickets[i] = srv.submit(tree, fam, q, k, radius, flags)
+
+        th = threading.Thread(target=producer)
+        th.start()
+        for i in range(count):
+            while tickets[i] is None:
                 time.sleep(0.0001)
-            tickets[i] = srv.submit(tree, fam, q, k, radius, flags)
+            lat[i] = srv.wait(tickets[i])[3]
+        end = time.perf_counter()
+        th.join()
+        return np.array(lat), end - start, arr
 
-    th = threading.Thread(target=producer)
-    th.start()
-    for i in range(len(reqs)):
-        while tickets[i] is None:
-            time.sleep(0.0001)
-        lat[i] = srv.wait(tickets[i])[3]
-    end = time.perf_counter()
-    th.join()
+    # rate sweep: delivered queries/s and p50/p99 latency as the offered load grows
+    sweep = []
+    for rate in (float(x) for x in args.rates.split(",") if x):
+        n = min(len(reqs), max(400, int(rate * 0.5)))
+        sl, sdt, sarr = run(rate, n)
+        nq = sum(r[2].shape[0] for r in reqs[:n])
+        sweep.append({"rate_per_s": rate, "requests": n, "offered_qps": round(nq / sarr[-1], 1),
+                      "delivered_qps": round(nq / sdt, 1), "p50_ms": round(float(np.percentile(sl, 50)), 3),
+                      "p99_ms": round(float(np.percentile(sl, 99)), 3)})
+        log(f"rate {rate:.0f}/s: {sweep[-1]}")
+    st0 = srv.stats()
+    lat, elapsed, arrivals = run(args.rate, len(reqs))
     st = srv.stats()
+    st = {k: st[k] - st0[k] if k != "peak_arena_bytes" else st[k] for k in st}
     srv.close()
-    lat = np.array(lat)
+    end, start = elapsed, 0.0
     by_fam = {}
     names = {ka.FAMILY_NEAREST_ONE: "nearest_one", ka.FAMILY_NEAREST_N: "nearest_n5",
              ka.FAMILY_BEST_N_WITHIN: "best_n_within5"}
@@ -534,8 +561,21 @@ def bench_serving(args, torch, ka):
         sel = np.array([r[1] == fam for r in reqs])
         by_fam[names[fam]] = {"requests": int(sel.sum()), "p50_ms": round(float(np.percentile(lat[sel], 50)), 3),
                               "p99_ms": round(float(np.percentile(lat[sel], 99)), 3)}
+    sizes = np.array([r[2].shape[0] for r in reqs])
+    by_size = {}
+    for lo, hi in ((1, 256), (256, 4096), (4096, 16384), (16384, 65537)):
+        sel = (sizes >= lo) & (sizes < hi)
+        if sel.any():
+            by_size[f"{lo}-{hi - 1}"] = {"requests": int(sel.sum()),
+                                         "p50_ms": round(float(np.percentile(lat[sel], 50)), 3),
+                                         "p99_ms": round(float(np.percentile(lat[sel], 99)), 3)}
     value = total_q / (end - start)
     log(f"serving: {value / 1e6:.1f} Mq/s, p50 {np.percentile(lat, 50):.2f} ms, p99 {np.percentile(lat, 99):.2f} ms, {st}")
+    log(f"by family {by_fam}; by size {by_size}")
+    slow = np.argsort(lat)[::-1][:12]
+    tail = [{"arrival_ms": round(float(arrivals[i]) * 1e3, 2), "latency_ms": round(float(lat[i]), 2),
+             "queries": int(sizes[i]), "family": names[reqs[i][1]]} for i in sorted(slow)]
+    log(f"slowest requests: {tail}")
     print(json.dumps({
         "metric": "queries/s and p50/p99 request latency, continuous batching over resident trees",
         "value": round(value, 1), "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -543,10 +583,11 @@ def bench_serving(args, torch, ka):
         "vs_baseline": None, "dtype": "f64 + u16(f64 output)", "data": "synthetic (seeded)",
         "config": {"workload": "serving-12M-f64-sphere+u16-palette", "requests": args.requests,
                    "rate_per_s": args.rate, "queries": int(total_q), "slots": 4,
-                   "arena_bytes": 1 << 30},
+                   "arena_bytes": 1 << 30, "max_part": args.max_part, "max_coalesce": 1 << 16},
+        "rate_sweep": sweep,
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "per_family": by_fam, "server": st,
+        "per_family": by_fam, "per_size": by_size, "slowest": tail, "server": st,
         "offered_qps": round(total_q / arrivals[-1], 1),
     }), flush=True)
 

@@ -296,11 +296,15 @@ int kd_forest_query_stats(kd_forest* forest, uint64_t* out4);
 /* ------------------------------------------------------------------- serving ---- */
 /* A continuous-batching request loop over resident trees (no Kiddo counterpart; it serves
  * the same families as KdTree::query). Requests carry HOST query arrays and HOST result
- * buffers and may be submitted from any thread. The loop admits requests in arrival order
- * when one of `slots` streams is free and the device arena (`arena_bytes`, mirrored in
- * pinned host memory) has room for the request's queries and results; it coalesces later
- * pending requests of the same kind (tree, family, k, radius, flags) into the same launch,
- * up to `max_coalesce` queries; launches on different slots overlap on the device. */
+ * buffers and may be submitted from any thread; each is cut into parts of at most
+ * `max_part` queries (kd_server_set_part; 16384 by default). The loop admits a part when
+ * one of `slots` streams is free and the device arena (`arena_bytes`, mirrored in pinned
+ * host memory) has room for its queries and results, taking first the oldest part of a
+ * request with the fewest parts on the device (a large request does not hold every slot
+ * while smaller ones wait); it coalesces pending parts of the same kind (tree, family, k,
+ * radius, flags), one per request, into the same launch, up to `max_coalesce` queries;
+ * launches on different slots overlap on the device. A request completes when its last
+ * part does; results never depend on the batching. */
 enum kd_family {
   KD_FAMILY_NEAREST_ONE = 1,   /* items[m], dists[m]                                   */
   KD_FAMILY_NEAREST_N = 2,     /* items[m*k], dists[m*k], counts[m]; radius as kd_nearest_n */
@@ -327,7 +331,10 @@ int kd_server_submit(kd_server* server, const kd_request* request, uint64_t* tic
 /* Blocks until the request is answered; returns its status and its latency (submit to
  * results in host memory). Each ticket is waited for once. */
 int kd_server_wait(kd_server* server, uint64_t ticket, double* latency_ms);
-/* out6 = {requests done, launches, queries launched, requests coalesced into another's
+/* Requests above max_part queries are cut into parts of max_part (0: never cut). Applies
+ * to requests submitted afterwards. */
+int kd_server_set_part(kd_server* server, uint64_t max_part);
+/* out6 = {requests done, launches, queries launched, parts coalesced into another part's
  * launch, admissions deferred for memory, peak arena bytes in use}. */
 int kd_server_stats(kd_server* server, uint64_t* out6);
 int kd_server_destroy(kd_server* server);

@@ -130,6 +130,7 @@ SIGNATURES = {
     "kd_server_submit": ([_vp, _vp, C.POINTER(_u64)], C.c_int),
     "kd_server_wait": ([_vp, _u64, C.POINTER(C.c_double)], C.c_int),
     "kd_server_stats": ([_vp, _vp], C.c_int),
+    "kd_server_set_part": ([_vp, _u64], C.c_int),
     "kd_server_destroy": ([_vp], C.c_int),
     "kd_tree_set_timing": ([_vp, _i32], C.c_int),
     "kd_last_kernel_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
@@ -684,7 +685,7 @@ class Server:
     """Continuous-batching request loop over resident trees (kd_server_* in kiddo_amd.h).
     Requests take host numpy queries; results come back as host numpy arrays."""
 
-    def __init__(self, trees, arena_bytes=256 << 20, slots=4, max_coalesce=1 << 18):
+    def __init__(self, trees, arena_bytes=256 << 20, slots=4, max_coalesce=1 << 18, max_part=None):
         self.trees = list(trees)
         arr = (C.c_void_p * len(self.trees))(*[t.handle.value for t in self.trees])
         h = C.c_void_p()
@@ -692,6 +693,8 @@ class Server:
                                       C.byref(h)), "kd_server_create")
         self._h = h
         self._live = {}
+        if max_part is not None:
+            _check(lib().kd_server_set_part(self._h, max_part), "kd_server_set_part")
 
     def submit(self, tree, family, queries, k=1, radius=math.inf, flags=0):
         t = self.trees[tree]

@@ -1,15 +1,19 @@
 // server.hip — continuous-batching request loop over resident trees.
 //
-// Requests (host query arrays and host result buffers) are submitted from any thread. A
-// single loop thread
-//   * admits pending requests in arrival order whenever a stream slot is free and the device
-//     result arena has room for the request's queries and results (admission by memory);
-//   * coalesces the admitted request with later pending requests of the same kind (same tree,
-//     family, k, radius and flags) into one launch, up to `max_coalesce` queries;
+// Requests (host query arrays and host result buffers) are submitted from any thread and
+// cut into parts of at most `max_part` queries. A single loop thread
+//   * admits pending parts whenever a stream slot is free and the device result arena has
+//     room for the part's queries and results (admission by memory), taking first the oldest
+//     part of a request with the fewest parts on the device, so that a large request cannot
+//     hold every slot while smaller ones wait;
+//   * coalesces the admitted part with other pending parts of the same kind (same tree,
+//     family, k, radius and flags), at most one part per request, into one launch of up to
+//     `max_coalesce` queries;
 //   * runs each launch on its slot's stream: host -> pinned staging -> HBM, the query kernel
 //     through the C ABI, HBM -> pinned staging, with an event at the end;
-//   * retires finished launches by polling their events, copies every request's slice of the
-//     results into its host buffers, frees the arena range and the slot, and stamps latency.
+//   * retires finished launches by polling their events, copies every part's slice of the
+//     results into its request's host buffers, frees the arena range and the slot, and stamps
+//     a request's latency when its last part is answered.
 // Launches on different slots overlap on the device; a request's results do not depend on
 // what it was batched with, because every query is answered independently.
 
@@ -40,6 +44,17 @@ struct Req {
   double latency_ms = 0;
   bool done = false;
   int status = KD_OK;
+  // loop thread only
+  uint32_t parts_left = 0;  // parts not answered yet
+  uint32_t in_flight = 0;   // parts on the device
+  uint64_t launch_mark = 0; // the launch being assembled holds one of its parts
+};
+
+// Queries [first, first + m) of a request: the unit of admission.
+struct Part {
+  Req* req;
+  uint64_t first, m;
+  bool admitted = false;
 };
 
 // First-fit allocator over one device arena (and its pinned mirror); host bookkeeping only.
@@ -88,8 +103,8 @@ struct Arena {
 
 struct Launch {
   int slot;
-  std::vector<Req*> reqs;
-  std::vector<uint64_t> q_begin;  // per request: first query index in the launch
+  std::vector<Part*> parts;
+  std::vector<uint64_t> q_begin;  // per part: first query index in the launch
   uint64_t m = 0, off = 0, bytes = 0;
   uint64_t q_bytes = 0, items_off = 0, dists_off = 0, counts_off = 0;
   uint32_t k = 0;  // result row width (1 for nearest_one)
@@ -104,6 +119,7 @@ struct kd_server {
   std::vector<kd_tree*> trees;
   uint32_t slots = 0;
   uint64_t max_coalesce = 0;
+  std::atomic<uint64_t> max_part{16384};
   uint8_t* dev = nullptr;
   uint8_t* pinned = nullptr;
   kd::Arena arena;
@@ -112,7 +128,7 @@ struct kd_server {
   std::vector<bool> slot_busy;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<kd::Req*> submitted;
+  std::deque<kd::Part*> submitted;
   std::map<uint64_t, kd::Req*> reqs;
   uint64_t next_ticket = 1;
   bool stop = false;
@@ -141,8 +157,8 @@ uint64_t launch_bytes(const kd_tree* t, const kd_request& r, uint64_t m) {
 }
 
 int run_launch(kd_server* S, Launch& L) {
-  kd_tree* t = S->trees[L.reqs[0]->r.tree];
-  const kd_request& r0 = L.reqs[0]->r;
+  const kd_request& r0 = L.parts[0]->req->r;
+  kd_tree* t = S->trees[r0.tree];
   hipStream_t s = S->streams[L.slot];
   auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
   L.k = (r0.family == KD_FAMILY_NEAREST_ONE || r0.family == KD_FAMILY_APPROX) ? 1 : r0.k;
@@ -151,10 +167,13 @@ int run_launch(kd_server* S, Launch& L) {
   L.items_off = L.off + al(L.q_bytes);
   L.dists_off = L.items_off + al(L.m * L.k * 4);
   L.counts_off = L.dists_off + al(L.m * L.k * L.o_size);
-  // gather the requests' queries into pinned staging, then one copy to HBM
+  // gather the parts' queries into pinned staging, then one copy to HBM
   const size_t row = t->dims * a_size(t);
-  for (size_t i = 0; i < L.reqs.size(); ++i)
-    std::memcpy(S->pinned + L.off + L.q_begin[i] * row, L.reqs[i]->r.queries, L.reqs[i]->r.m * row);
+  for (size_t i = 0; i < L.parts.size(); ++i) {
+    const Part* P = L.parts[i];
+    std::memcpy(S->pinned + L.off + L.q_begin[i] * row,
+                static_cast<const uint8_t*>(P->req->r.queries) + P->first * row, P->m * row);
+  }
   KD_HIP(hipMemcpyAsync(S->dev + L.off, S->pinned + L.off, L.q_bytes, hipMemcpyHostToDevice, s));
   void* q = S->dev + L.off;
   uint32_t* items = reinterpret_cast<uint32_t*>(S->dev + L.items_off);
@@ -180,31 +199,41 @@ int run_launch(kd_server* S, Launch& L) {
 
 void finish_launch(kd_server* S, Launch& L, int status) {
   const auto now = Clock::now();
-  for (size_t i = 0; i < L.reqs.size(); ++i) {
-    Req* R = L.reqs[i];
-    const uint64_t b = L.q_begin[i], m = R->r.m;
+  std::vector<Req*> finished;
+  for (size_t i = 0; i < L.parts.size(); ++i) {
+    Part* P = L.parts[i];
+    Req* R = P->req;
+    const uint64_t b = L.q_begin[i], m = P->m, f = P->first;
     if (status == KD_OK) {
-      if (R->r.items) std::memcpy(R->r.items, S->pinned + L.items_off + b * L.k * 4, m * L.k * 4);
+      if (R->r.items) std::memcpy(R->r.items + f * L.k, S->pinned + L.items_off + b * L.k * 4, m * L.k * 4);
       if (R->r.dists)
-        std::memcpy(R->r.dists, S->pinned + L.dists_off + b * L.k * L.o_size, m * L.k * L.o_size);
+        std::memcpy(static_cast<uint8_t*>(R->r.dists) + f * L.k * L.o_size,
+                    S->pinned + L.dists_off + b * L.k * L.o_size, m * L.k * L.o_size);
       if (R->r.counts && L.k > 1 && R->r.family != KD_FAMILY_NEAREST_ONE)
-        std::memcpy(R->r.counts, S->pinned + L.counts_off + b * 4, m * 4);
+        std::memcpy(R->r.counts + f, S->pinned + L.counts_off + b * 4, m * 4);
+    } else if (R->status == KD_OK) {
+      R->status = status;
     }
-    R->status = status;
-    R->latency_ms = std::chrono::duration<double, std::milli>(now - R->submitted).count();
+    --R->in_flight;
+    if (--R->parts_left == 0) {
+      R->latency_ms = std::chrono::duration<double, std::milli>(now - R->submitted).count();
+      finished.push_back(R);
+    }
+    delete P;
   }
   std::lock_guard<std::mutex> g(S->mu);
   S->arena.release(L.off, L.bytes);
   S->slot_busy[L.slot] = false;
-  for (Req* R : L.reqs) R->done = true;
-  S->st_done += L.reqs.size();
+  for (Req* R : finished) R->done = true;  // R may be deleted by its waiter from here on
+  S->st_done += finished.size();
   S->cv_done.notify_all();
 }
 
 void serve(kd_server* S) {
   (void)hipSetDevice(S->device);
   std::vector<Launch> inflight;
-  std::deque<Req*> pending;
+  std::deque<Part*> pending;
+  uint64_t serial = 0;
   for (;;) {
     {
       std::unique_lock<std::mutex> g(S->mu);
@@ -238,33 +267,42 @@ void serve(kd_server* S) {
           }
       }
       if (slot < 0) break;
-      Req* head = pending.front();
-      kd_tree* t = S->trees[head->r.tree];
+      // the head: the oldest part among the requests with the fewest parts on the device
+      auto hit = pending.begin();
+      for (auto it = pending.begin(); it != pending.end(); ++it)
+        if ((*it)->req->in_flight < (*hit)->req->in_flight) hit = it;
+      Part* head = *hit;
+      const kd_request& hr = head->req->r;
+      kd_tree* t = S->trees[hr.tree];
       Launch L;
       L.slot = slot;
-      L.reqs.push_back(head);
+      L.parts.push_back(head);
       L.q_begin.push_back(0);
-      L.m = head->r.m;
-      for (auto it = std::next(pending.begin()); it != pending.end(); ++it) {
-        if (!same_kind((*it)->r, head->r)) continue;
-        if (L.m + (*it)->r.m > S->max_coalesce) break;
+      L.m = head->m;
+      head->req->launch_mark = ++serial;
+      // other parts of the same kind, one per request, in arrival order
+      for (auto it = pending.begin(); it != pending.end(); ++it) {
+        Part* P = *it;
+        if (P->req->launch_mark == serial || !same_kind(P->req->r, hr)) continue;
+        if (L.m + P->m > S->max_coalesce) continue;
+        P->req->launch_mark = serial;
         L.q_begin.push_back(L.m);
-        L.reqs.push_back(*it);
-        L.m += (*it)->r.m;
+        L.parts.push_back(P);
+        L.m += P->m;
       }
       // shrink the group until it fits the arena; the head alone must eventually fit
       bool ok = false;
       {
         std::lock_guard<std::mutex> g(S->mu);
-        while (!L.reqs.empty()) {
-          L.bytes = launch_bytes(t, head->r, L.m);
+        while (!L.parts.empty()) {
+          L.bytes = launch_bytes(t, hr, L.m);
           if (S->arena.alloc(L.bytes, L.off)) {
             ok = true;
             break;
           }
-          if (L.reqs.size() == 1) break;
-          L.m -= L.reqs.back()->r.m;
-          L.reqs.pop_back();
+          if (L.parts.size() == 1) break;
+          L.m -= L.parts.back()->m;
+          L.parts.pop_back();
           L.q_begin.pop_back();
         }
         if (ok) {
@@ -274,12 +312,17 @@ void serve(kd_server* S) {
       }
       if (!ok) {
         ++S->st_mem_waits;
-        break;  // wait for memory to free up (head of line keeps arrival order)
+        break;  // wait for memory to free up
       }
-      for (Req* R : L.reqs) pending.erase(std::find(pending.begin(), pending.end(), R));
+      for (Part* P : L.parts) {
+        P->admitted = true;
+        ++P->req->in_flight;
+      }
+      pending.erase(std::remove_if(pending.begin(), pending.end(), [](const Part* P) { return P->admitted; }),
+                    pending.end());
       ++S->st_launches;
       S->st_queries += L.m;
-      S->st_coalesced += L.reqs.size() - 1;
+      S->st_coalesced += L.parts.size() - 1;
       const int rc = run_launch(S, L);
       if (rc) {
         finish_launch(S, L, rc);
@@ -349,16 +392,20 @@ int kd_server_submit(kd_server* S, const kd_request* r, uint64_t* ticket) {
     return kd::set_error(KD_ERR_INVALID, "k must be >= 1");
   if (r->family < KD_FAMILY_NEAREST_ONE || r->family > KD_FAMILY_APPROX)
     return kd::set_error(KD_ERR_UNSUPPORTED, "family %d is not served", r->family);
-  if (kd::launch_bytes(S->trees[r->tree], *r, r->m) > S->arena.size)
-    return kd::set_error(KD_ERR_CAPACITY, "request needs more result memory than the arena holds");
+  const uint64_t part = S->max_part ? std::min<uint64_t>(S->max_part, r->m) : r->m;
+  if (kd::launch_bytes(S->trees[r->tree], *r, part) > S->arena.size)
+    return kd::set_error(KD_ERR_CAPACITY, "a part of the request needs more memory than the arena holds");
   auto* R = new kd::Req;
   R->r = *r;
   R->submitted = kd::Clock::now();
+  std::vector<kd::Part*> parts;
+  for (uint64_t f = 0; f < r->m; f += part) parts.push_back(new kd::Part{R, f, std::min(part, r->m - f)});
+  R->parts_left = uint32_t(parts.size());
   {
     std::lock_guard<std::mutex> g(S->mu);
     R->ticket = S->next_ticket++;
     S->reqs[R->ticket] = R;
-    S->submitted.push_back(R);
+    for (kd::Part* P : parts) S->submitted.push_back(P);
   }
   S->cv_work.notify_one();
   *ticket = R->ticket;
@@ -377,6 +424,13 @@ int kd_server_wait(kd_server* S, uint64_t ticket, double* latency_ms) {
   S->reqs.erase(it);
   delete R;
   return status;
+}
+
+int kd_server_set_part(kd_server* S, uint64_t max_part) {
+  if (!S) return kd::set_error(KD_ERR_INVALID, "server is null");
+  if (max_part > 0xFFFFFFFFull) return kd::set_error(KD_ERR_INVALID, "parts hold < 2^32 queries");
+  S->max_part = max_part;
+  return KD_OK;
 }
 
 int kd_server_stats(kd_server* S, uint64_t* out6) {

@@ -130,3 +130,36 @@ def test_serving_rejects_oversized_request(ka, serving_case):
     with pytest.raises(ka.KiddoAmdError):
         srv.submit(0, ka.FAMILY_NEAREST_N, latlon_clusters(100_000, seed=1), 16)
     srv.close()
+
+
+def test_serving_parts_equal_alone_and_oracle(ka, serving_case):
+    """Requests cut into parts of 700 queries (and a request of exactly one part, and one of
+    a part plus one query) are answered exactly as whole requests alone, and as the oracle;
+    the server launches more parts than it receives requests."""
+    from kiddo_amd.datasets import latlon_clusters, palette_u16
+    trees, refs, _ = serving_case
+    reqs = []
+    for i, m in enumerate((5000, 700, 701, 1, 2800, 6123)):
+        reqs.append((0, ka.FAMILY_NEAREST_N, latlon_clusters(m, seed=400 + i), 5, np.inf))
+        reqs.append((1, ka.FAMILY_BEST_N_WITHIN, palette_u16(m, seed=500 + i), 5, 4.0e6))
+        reqs.append((0, ka.FAMILY_NEAREST_ONE, latlon_clusters(m, seed=600 + i), 1, np.inf))
+    srv1 = ka.Server(trees, arena_bytes=64 << 20, slots=1, max_coalesce=1, max_part=0)
+    alone = _run_alone(ka, srv1, reqs)
+    srv1.close()
+    srv = ka.Server(trees, arena_bytes=64 << 20, slots=4, max_coalesce=4000, max_part=700)
+    mixed = _run_mixed(ka, srv, reqs)
+    st = srv.stats()
+    srv.close()
+    parts = sum(-(-r[2].shape[0] // 700) for r in reqs)
+    assert st["done"] == len(reqs) and st["launches"] + st["coalesced"] == parts, st
+    for i, ((tree, fam, q, k, r), a, b) in enumerate(zip(reqs, alone, mixed)):
+        _same(a, b, f"request {i} parts vs whole")
+        o = refs[tree]
+        if fam == ka.FAMILY_NEAREST_ONE:
+            oi, od = o.nearest_one(q)
+            assert_same_bits(b[0], oi, f"req {i} nearest_one items")
+            assert_same_bits(b[1], od, f"req {i} nearest_one dists")
+        elif fam == ka.FAMILY_NEAREST_N:
+            _same(b, o.nearest_n(q, k, r), f"req {i} nearest_n")
+        else:
+            _same(b, o.best_n_within(q, r, k), f"req {i} best_n_within")
