@@ -624,32 +624,59 @@ def bench_mixture(args, torch, ka):
     dt, kms, _ = _time_knn(tree, q, 16, flags, args.steps, args.warmup, torch)
     st = tree.query_stats()
     log(f"mixture knn16: {dt * 1e3:.1f} ms/step ({m / dt / 1e6:.1f} Mq/s), kernel {kms:.1f} ms, {st}")
-    # within, chunked by result memory
+    # within, chunked by result memory; launch shapes of both passes autotuned on a sample
     r2 = 0.005 ** 2
     wflags = ka.KD_UNSORTED | ka.KD_MORTON
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    offs = torch.empty(m + 1, dtype=torch.int64, device="cuda")
-    ka._check(ka.lib().kd_within_count(tree.handle, q.data_ptr(), m, r2, wflags, offs.data_ptr(),
-                                       ka._stream_ptr(None)), "kd_within_count")
-    offs_h = offs.cpu().numpy()
-    t_count = time.perf_counter() - t0
-    total = int(offs_h[-1])
+    wshape = tree.autotune_within(q, r2, wflags)
+    log(f"autotuned within shapes: {wshape}")
     cap = max(1, args.result_budget // 8)  # u32 item + f32 distance per result
-    items = torch.empty(min(cap, max(total, 1)), dtype=torch.uint32, device="cuda")
-    dists = torch.empty(min(cap, max(total, 1)), dtype=torch.float32, device="cuda")
-    a, chunks = 0, 0
-    while a < m:
-        b = int(np.searchsorted(offs_h, offs_h[a] + cap, side="right")) - 1
-        b = max(min(b, m), a + 1)
-        sub = (offs[a:b + 1] - offs[a]).contiguous()
-        ka._check(ka.lib().kd_within_fill(tree.handle, q[a:b].data_ptr(), b - a, r2, wflags,
-                                          sub.data_ptr(), items.data_ptr(), dists.data_ptr(),
-                                          ka._stream_ptr(None)), "kd_within_fill")
-        chunks += 1
-        a = b
-    torch.cuda.synchronize()
-    t_within = time.perf_counter() - t0
+    items = torch.empty(cap, dtype=torch.uint32, device="cuda")
+    dists = torch.empty(cap, dtype=torch.float32, device="cuda")
+
+    def within_pass(qq):
+        """count pass over the batch, then fill passes over chunks that each fit the result
+        budget; chunks are contiguous ranges of the batch's Morton order (kd_morton_order),
+        so each fill launch sees spatially coherent queries, and each chunk's results are a
+        CSR over its queries in that order (query perm[j] of the batch for row j - a)"""
+        mm = qq.shape[0]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        offs = torch.empty(mm + 1, dtype=torch.int64, device="cuda")
+        ka._check(ka.lib().kd_within_count(tree.handle, qq.data_ptr(), mm, r2, wflags, offs.data_ptr(),
+                                           ka._stream_ptr(None)), "kd_within_count")
+        count_kernel_ms = tree.last_kernel_ms()
+        st = tree.query_stats()
+        t_count = time.perf_counter() - t0
+        perm = tree.morton_order(qq).view(torch.int32).long()
+        cum = torch.zeros(mm + 1, dtype=torch.int64, device="cuda")
+        torch.cumsum((offs[1:] - offs[:-1])[perm], 0, out=cum[1:])
+        cum_h = cum.cpu().numpy()
+        tot = int(cum_h[-1])
+        a, chunks, fill_kernel_ms = 0, 0, 0.0
+        while a < mm:
+            b = int(np.searchsorted(cum_h, cum_h[a] + cap, side="right")) - 1
+            b = max(min(b, mm), a + 1)
+            sub = (cum[a:b + 1] - cum[a]).contiguous()
+            qs = qq.index_select(0, perm[a:b])
+            ka._check(ka.lib().kd_within_fill(tree.handle, qs.data_ptr(), b - a, r2, wflags,
+                                              sub.data_ptr(), items.data_ptr(), dists.data_ptr(),
+                                              ka._stream_ptr(None)), "kd_within_fill")
+            fill_kernel_ms += tree.last_kernel_ms()
+            chunks += 1
+            a = b
+        torch.cuda.synchronize()
+        return dict(t_within=time.perf_counter() - t0, t_count=t_count, count_kernel_ms=count_kernel_ms,
+                    fill_kernel_ms=fill_kernel_ms, total=tot, chunks=chunks, stats=st)
+
+    within_pass(q[:200_000])  # warm-up: first launches of the pipeline's kernels
+    W = within_pass(q)
+    t_within, t_count, total, chunks = W["t_within"], W["t_count"], W["total"], W["chunks"]
+    count_kernel_ms, fill_kernel_ms, wst = W["count_kernel_ms"], W["fill_kernel_ms"], W["stats"]
+    # algorithmic bytes: every entry scanned (3 f32 coordinates), the item of every result
+    # (read) and the result itself (u32 item + f32 distance, written), stems read, queries;
+    # the count pass reads the same entries and stems and writes one u64 count per query
+    fill_bytes = wst["points"] * 12 + total * (4 + 8) + wst["stems"] * 4 + m * 12
+    count_bytes = wst["points"] * 12 + wst["stems"] * 4 + m * (12 + 8)
     log(f"mixture within: {total} results in {chunks} chunks, count {t_count * 1e3:.0f} ms, "
         f"total {t_within * 1e3:.0f} ms ({total / t_within / 1e9:.2f} G results/s)")
     algo = st["points"] * 16 + st["stems"] * 4 + m * 12 + m * 16 * 8
@@ -662,9 +689,25 @@ def bench_mixture(args, torch, ka):
                    "k": 16, "launch": chosen},
         "within_r0.005_unsorted": {"results": total, "chunks": chunks,
                                    "result_budget_bytes": args.result_budget,
+                                   "launch": wshape,
                                    "count_ms": round(t_count * 1e3, 1),
+                                   "count_kernel_ms": round(count_kernel_ms, 2),
+                                   "fill_kernel_ms": round(fill_kernel_ms, 2),
                                    "total_ms": round(t_within * 1e3, 1),
-                                   "results_per_s": round(total / t_within, 1)},
+                                   "results_per_s": round(total / t_within, 1),
+                                   "entries_per_query": round(wst["points"] / m, 1),
+                                   "roofline_fill": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                                                     "achieved": round(fill_bytes / (fill_kernel_ms / 1e3) / 1e9, 1),
+                                                     "frac": round(fill_bytes / (fill_kernel_ms / 1e3) / 1e9
+                                                                   / HBM_PEAK_GBS, 4),
+                                                     "algo_bytes": int(fill_bytes), "traffic": None,
+                                                     "kernel": "k_within_fill_group"},
+                                   "roofline_count": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                                                      "achieved": round(count_bytes / (count_kernel_ms / 1e3) / 1e9, 1),
+                                                      "frac": round(count_bytes / (count_kernel_ms / 1e3) / 1e9
+                                                                    / HBM_PEAK_GBS, 4),
+                                                      "algo_bytes": int(count_bytes), "traffic": None,
+                                                      "kernel": "k_within_count_group"}},
         "roofline": {"bound": "hbm", "achieved": round(algo / (kms / 1e3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(algo / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,

@@ -162,6 +162,19 @@ int kd_tree_tune(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_t
 int kd_tree_tune2(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_t lds_levels,
                   int32_t waves);
 
+/* Launch shapes of the two within passes (no Kiddo counterpart; results never depend on
+ * them): 1 one thread per query, 4/8/16/32 lanes per query sharing each leaf visit
+ * (SquaredEuclidean; other metrics use one thread per query), 0 automatic (the count pass:
+ * one thread per query; the fill pass: 16 lanes per query when the batch averages 32 or
+ * more results per query, else one thread). */
+int kd_tree_tune_within(kd_tree* tree, int32_t count_group, int32_t fill_group);
+/* Autotuner of the within passes: times each shape of the count pass on up to 2^20 of the
+ * queries (with KD_MORTON: 16 runs of their Morton order), then each shape of the fill pass
+ * on as many of them as 2^28 results allow, and keeps the fastest of each in the tree
+ * (chosen2 = {count shape, fill shape}). Synchronous. */
+int kd_autotune_within(kd_tree* tree, const void* queries, uint64_t m, double radius, uint32_t flags,
+                       int32_t* chosen2, void* stream);
+
 /* Autotuner: times each candidate launch shape (thread vs group per query, group size,
  * LDS stem levels, occupancy floor) on up to 2^20 of the given queries (with KD_MORTON:
  * 16 contiguous runs of their Morton order) with the given k/radius/flags, keeps the
@@ -203,6 +216,13 @@ int kd_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k, dou
 #define KD_NOALLOC_MAX_QUERIES (1u << 20)
 int kd_nearest_n_noalloc(kd_tree* tree, const void* queries, uint64_t m, uint32_t k, double radius,
                          uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream);
+
+/* The order KD_MORTON visits a batch in (no Kiddo counterpart): perm[m] receives the query
+ * indices sorted by the Morton key of their coordinates over the tree's bounding box. A
+ * caller that cuts a large batch into chunks (for example within results bounded by result
+ * memory) can take contiguous ranges of this order, so that every chunk stays spatially
+ * coherent. Device pointers; stream-ordered. */
+int kd_morton_order(kd_tree* tree, const void* queries, uint64_t m, uint32_t* perm, void* stream);
 
 /* tree.query(q).within(radius)[.unsorted()][.exclusive_boundaries()], phase 1
  * (query/within.rs:18-41, within_unsorted.rs:133-156): counts results per query and

@@ -103,12 +103,15 @@ SIGNATURES = {
     "kd_tree_get_info": ([_vp, C.POINTER(TreeInfo)], C.c_int),
     "kd_tree_tune": ([_vp, _i32, _i32, _i32], C.c_int),
     "kd_tree_tune2": ([_vp, _i32, _i32, _i32, _i32], C.c_int),
+    "kd_tree_tune_within": ([_vp, _i32, _i32], C.c_int),
+    "kd_autotune_within": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp], C.c_int),
     "kd_autotune_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp], C.c_int),
     "kd_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_approx_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_nearest_n_noalloc": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_within_count": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp], C.c_int),
+    "kd_morton_order": ([_vp, _vp, _u64, _vp, _vp], C.c_int),
     "kd_within_fill": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_best_n_within": ([_vp, _vp, _u64, _f64, _u32, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_periodic_nearest_one": ([_vp, _vp, _u64, _vp, _u32, _vp, _vp, _vp], C.c_int),
@@ -413,6 +416,19 @@ class ImmutableKdTree:
         _check(lib().kd_tree_tune2(self._h, knn_variant, group_size, lds_levels, waves),
                "kd_tree_tune2")
 
+    def tune_within(self, count_group: int = 0, fill_group: int = 0):
+        """Launch shapes of the within passes: 0 automatic, 1 thread per query, 4..32 lanes
+        per query."""
+        _check(lib().kd_tree_tune_within(self._h, count_group, fill_group), "kd_tree_tune_within")
+
+    def autotune_within(self, q, radius, flags=0, stream=None):
+        """Times the within passes' launch shapes on a sample of q and keeps the fastest."""
+        q = self._queries(q)
+        chosen = np.zeros(2, np.int32)
+        _check(lib().kd_autotune_within(self._h, q.data_ptr(), q.shape[0], float(radius), flags,
+                                        _np_ptr(chosen), _stream_ptr(stream)), "kd_autotune_within")
+        return dict(count_group=int(chosen[0]), fill_group=int(chosen[1]))
+
     def autotune_nearest_n(self, q, k, radius=math.inf, flags=0, stream=None):
         """Times the candidate launch shapes on a sample of q and keeps the fastest."""
         q = self._queries(q)
@@ -461,6 +477,15 @@ class ImmutableKdTree:
             if (not isinstance(t, torch.Tensor) or not t.is_cuda or tuple(t.shape) != shape
                     or t.dtype != dt or not t.is_contiguous()):
                 raise ValueError(f"out.{name} must be a contiguous cuda {dt} tensor of shape {shape}")
+
+    def morton_order(self, q, stream=None):
+        """The permutation KD_MORTON visits q in (a cuda uint32 tensor of q's indices)."""
+        torch = _torch()
+        q = self._queries(q)
+        perm = torch.empty(q.shape[0], dtype=torch.uint32, device="cuda")
+        _check(lib().kd_morton_order(self._h, q.data_ptr(), q.shape[0], perm.data_ptr(), _stream_ptr(stream)),
+               "kd_morton_order")
+        return perm
 
     def nearest_one_raw(self, q, flags=0, stream=None, approx=False):
         torch = _torch()

@@ -719,6 +719,16 @@ int kd_tree_tune2(kd_tree* t, int32_t knn_variant, int32_t group_size, int32_t l
   return KD_OK;
 }
 
+int kd_tree_tune_within(kd_tree* t, int32_t count_group, int32_t fill_group) {
+  if (int rc = kd::check_tree(t)) return rc;
+  auto ok = [](int g) { return g == 0 || g == 1 || g == 4 || g == 8 || g == 16 || g == 32; };
+  if (!ok(count_group) || !ok(fill_group))
+    return kd::set_error(KD_ERR_INVALID, "within launch shapes must be 0, 1, 4, 8, 16 or 32");
+  t->count_group = count_group;
+  t->fill_group = fill_group;
+  return KD_OK;
+}
+
 namespace kd {
 namespace {
 // Metric field of the flags (KD_METRIC_*): kind 0..3; Minkowski exponents 1 and 2 are
@@ -812,6 +822,32 @@ int kd_nearest_n_noalloc(kd_tree* t, const void* queries, uint64_t m, uint32_t k
   }
   const kd::QueryArgs a{t, queries, m, order.perm, flags, s};
   return kd::launch_nearest_n(a, int(k), radius, items, dists, counts);
+}
+
+__global__ void kd_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * kd::dev::kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+int kd_morton_order(kd_tree* t, const void* queries, uint64_t m, uint32_t* perm, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (m == 0) return KD_OK;
+  if (!queries || !perm) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  kd::Order order;
+  int orc = KD_OK;
+  KD_DISPATCH_AK(t->dtype, t->dims,
+                 orc = (kd::make_order<A, KK>(t, static_cast<const A*>(queries), m, KD_MORTON, s, order)));
+  if (orc) return orc;
+  if (!order.perm) {  // a single query
+    const uint32_t zero = 0;
+    KD_HIP(hipMemcpyAsync(perm, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    KD_HIP(hipStreamSynchronize(s));
+    return KD_OK;
+  }
+  hipLaunchKernelGGL(kd_copy_u32, dim3(kd::grid_for(m)), dim3(kd::dev::kBlock), 0, s, order.perm, perm, m);
+  KD_HIP(hipGetLastError());
+  return KD_OK;
 }
 
 int kd_within_count(kd_tree* t, const void* queries, uint64_t m, double radius, uint32_t flags,
@@ -960,6 +996,48 @@ int kd_periodic_within_fill(kd_tree* t, const void* queries, uint64_t m, const v
   return kd::launch_periodic_within_fill(a, box, radius, offsets, items, dists);
 }
 
+}  // extern "C"
+
+namespace kd {
+namespace {
+// The autotuners' sample of ms queries of a batch: the first ms queries, or, for a
+// Morton-ordered batch, 16 evenly spaced runs of its own Morton order (its first queries in
+// input order are spatially sparse and tune for the wrong shape). *mem (freed by the
+// caller with hipFree) holds a gathered sample.
+int tuning_sample(kd_tree* t, const void* queries, uint64_t m, uint64_t ms, uint32_t flags, hipStream_t s,
+                  void** mem, const void** sample) {
+  constexpr uint64_t kRuns = 16;
+  *mem = nullptr;
+  *sample = queries;
+  if (!((flags & KD_MORTON) && m > ms && ms % kRuns == 0)) return KD_OK;
+  int src = KD_OK;
+  KD_DISPATCH_AK(t->dtype, t->dims, {
+    Order order;
+    src = make_order<A, KK>(t, static_cast<const A*>(queries), m, flags, s, order);
+    if (src == KD_OK && order.perm) {
+      if (hipMalloc(mem, ms * KK * sizeof(A)) != hipSuccess) {
+        *mem = nullptr;
+        src = set_error(KD_ERR_NO_MEMORY, "hipMalloc failed");
+      } else {
+        hipLaunchKernelGGL((dev::k_gather_sample<A, KK>), dim3(grid_for(ms)), dim3(dev::kBlock), 0, s,
+                           static_cast<const A*>(queries), order.perm, m, ms, kRuns, static_cast<A*>(*mem));
+        if (hipGetLastError() != hipSuccess) src = set_error(KD_ERR_DEVICE, "sample gather failed");
+        (void)hipStreamSynchronize(s);
+        *sample = *mem;
+      }
+    }
+  });
+  if (src != KD_OK) {
+    (void)hipFree(*mem);
+    *mem = nullptr;
+  }
+  return src;
+}
+}  // namespace
+}  // namespace kd
+
+extern "C" {
+
 int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t k, double radius,
                           uint32_t flags, int32_t* chosen4, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;
@@ -978,34 +1056,9 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
   if (flags & KD_SCREEN_F32)
     for (int w = 4; w <= 6; ++w) cands.push_back({1, 0, -1, w});
   if (g0 < 64) cands.push_back({2, 2 * g0, -1, 0});
-  // A Morton-ordered batch is sampled along its own Morton order (16 runs), not by its
-  // first queries in input order, which are spatially sparse and tune for the wrong shape.
   void* sample_mem = nullptr;
   const void* sample = queries;
-  constexpr uint64_t kRuns = 16;
-  if ((flags & KD_MORTON) && m > ms && ms % kRuns == 0) {
-    int src = KD_OK;
-    KD_DISPATCH_AK(t->dtype, t->dims, {
-      kd::Order order;
-      src = kd::make_order<A, KK>(t, static_cast<const A*>(queries), m, flags, s, order);
-      if (src == KD_OK && order.perm) {
-        if (hipMalloc(&sample_mem, ms * KK * sizeof(A)) != hipSuccess) {
-          sample_mem = nullptr;
-          src = kd::set_error(KD_ERR_NO_MEMORY, "hipMalloc failed");
-        } else {
-          hipLaunchKernelGGL((kd::dev::k_gather_sample<A, KK>), dim3(kd::grid_for(ms)), dim3(kd::dev::kBlock), 0, s,
-                             static_cast<const A*>(queries), order.perm, m, ms, kRuns, static_cast<A*>(sample_mem));
-          if (hipGetLastError() != hipSuccess) src = kd::set_error(KD_ERR_DEVICE, "sample gather failed");
-          (void)hipStreamSynchronize(s);
-          sample = sample_mem;
-        }
-      }
-    });
-    if (src != KD_OK) {
-      (void)hipFree(sample_mem);
-      return src;
-    }
-  }
+  if (int src = kd::tuning_sample(t, queries, m, ms, flags, s, &sample_mem, &sample)) return src;
   void* mem = nullptr;
   const size_t osz = ms * k * (sizeof(uint32_t) + sizeof(double)) + ms * sizeof(uint32_t) + 1024;
   if (hipMalloc(&mem, osz) != hipSuccess) {
@@ -1057,6 +1110,97 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
     chosen4[1] = best.group;
     chosen4[2] = best.lds;
     chosen4[3] = best.waves;
+  }
+  return KD_OK;
+}
+
+int kd_autotune_within(kd_tree* t, const void* queries, uint64_t m, double radius, uint32_t flags,
+                       int32_t* chosen2, void* stream) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (!queries) return kd::set_error(KD_ERR_INVALID, "queries is null");
+  if (int rc = kd::check_radius(radius)) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t ms = std::min<uint64_t>(m, uint64_t(1) << 20);
+  if (ms == 0) return KD_OK;
+  void* sample_mem = nullptr;
+  const void* sample = queries;
+  if (int src = kd::tuning_sample(t, queries, m, ms, flags, s, &sample_mem, &sample)) return src;
+  const int save_count = t->count_group, save_fill = t->fill_group;
+  const bool save_timing = t->timing;
+  uint64_t* offs = nullptr;
+  uint32_t* items = nullptr;
+  void* dists = nullptr;
+  int rc = kd_tree_set_timing(t, 1);
+  if (rc == KD_OK && hipMalloc(&offs, (ms + 1) * sizeof(uint64_t)) != hipSuccess) {
+    offs = nullptr;
+    rc = kd::set_error(KD_ERR_NO_MEMORY, "hipMalloc failed");
+  }
+  const int shapes[] = {1, 4, 8, 16, 32};
+  // the count pass: each shape twice, the second timed
+  float best = 1e30f;
+  int best_count = 1;
+  for (int g : shapes) {
+    if (rc) break;
+    t->count_group = g;
+    float ms_c = 0.0f;
+    for (int rep = 0; rep < 2 && rc == KD_OK; ++rep) {
+      rc = kd_within_count(t, sample, ms, radius, flags, offs, stream);
+      if (rc == KD_OK) rc = kd_last_kernel_ms(t, &ms_c);
+    }
+    if (rc == KD_OK && ms_c < best) {
+      best = ms_c;
+      best_count = g;
+    }
+  }
+  // the fill pass, on as many of the sample's queries as 2^28 results allow
+  int best_fill = 1;
+  uint64_t mf = ms, total = 0;
+  std::vector<uint64_t> h(ms + 1);
+  if (rc == KD_OK) {
+    t->count_group = best_count;
+    if (hipMemcpy(h.data(), offs, (ms + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = kd::set_error(KD_ERR_DEVICE, "offsets copy failed");
+  }
+  if (rc == KD_OK) {
+    constexpr uint64_t kBudget = uint64_t(1) << 28;
+    mf = uint64_t(std::upper_bound(h.begin(), h.end(), kBudget) - h.begin()) - 1;
+    mf = std::max<uint64_t>(std::min<uint64_t>(mf, ms), 1);
+    total = h[mf];
+    const size_t osz = t->dtype == KD_F32 ? 4 : 8;
+    if (hipMalloc(&items, std::max<uint64_t>(total, 1) * 4) != hipSuccess ||
+        hipMalloc(&dists, std::max<uint64_t>(total, 1) * osz) != hipSuccess)
+      rc = kd::set_error(KD_ERR_NO_MEMORY, "hipMalloc failed");
+  }
+  best = 1e30f;
+  for (int g : shapes) {
+    if (rc) break;
+    t->fill_group = g;
+    float ms_c = 0.0f;
+    for (int rep = 0; rep < 2 && rc == KD_OK; ++rep) {
+      rc = kd_within_fill(t, sample, mf, radius, flags, offs, items, dists, stream);
+      if (rc == KD_OK) rc = kd_last_kernel_ms(t, &ms_c);
+    }
+    if (rc == KD_OK && ms_c < best) {
+      best = ms_c;
+      best_fill = g;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(offs);
+  (void)hipFree(items);
+  (void)hipFree(dists);
+  (void)hipFree(sample_mem);
+  t->timing = save_timing;
+  if (rc) {
+    t->count_group = save_count;
+    t->fill_group = save_fill;
+    return rc;
+  }
+  t->count_group = best_count;
+  t->fill_group = best_fill;
+  if (chosen2) {
+    chosen2[0] = best_count;
+    chosen2[1] = best_fill;
   }
   return KD_OK;
 }

@@ -1308,8 +1308,8 @@ template <class O, int G> struct GroupWithinVis {
   __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
   __device__ bool unbounded() const { return radius == oinf<O>(); }
   template <class A, int K, class QW>
-  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
-    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl = 1u) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
     const int base = int(threadIdx.x & 63u) - gl;  // first lane of this group in the wave
     for (uint32_t p0 = s; p0 < e; p0 += G) {
       const uint32_t p = p0 + uint32_t(gl);

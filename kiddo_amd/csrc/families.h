@@ -209,8 +209,38 @@ __global__ __launch_bounds__(kBlock) void k_within_fill(DevTree<A, K> t, const A
   flush_counters(cnt, stats);
 }
 
-// within fill, G lanes per query (dense outputs; SquaredEuclidean, the group traversal's
-// metric). The count comes from kd_within_count as for the thread form.
+// within count, G lanes per query (SquaredEuclidean): the group form of k_within_count, as
+// k_within_fill_group below without writes; lane 0 of each group stores the count.
+template <class A, int K, int G>
+__global__ __launch_bounds__(kBlock) void k_within_count_group(DevTree<A, K> t, const A* __restrict__ queries,
+                                                               const uint32_t* __restrict__ order, uint64_t m,
+                                                               typename Ax<A>::O radius, int excl,
+                                                               uint64_t* __restrict__ counts,
+                                                               unsigned long long* __restrict__ stats) {
+  using O = typename Ax<A>::O;
+  pin_stems(t);
+  const uint64_t gid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / G;
+  const int gl = int(threadIdx.x) & (G - 1);
+  const bool active = gid < m;  // uniform within a group
+  Counters c;
+  if (active) {
+    const uint64_t qi = order ? order[gid] : gid;
+    Small<A, K> q;
+    Small<O, K> qw;
+    load_query<A, K>(queries, qi, q, qw);
+    GroupWithinVis<O, G> v{radius, excl != 0, gl, 0, nullptr, nullptr, 0};
+    traverse(t, q, qw, v, c);
+    if (gl == 0) counts[qi] = v.count;
+    if (gl != 0) c = Counters{};  // counted once per group
+  }
+  flush_counters(c, stats);
+}
+
+// within fill, G lanes per query (dense outputs; SquaredEuclidean). Every lane of a group
+// runs the thread traversal of its query: with the same query and the radius as the only
+// bound, the lanes take the same path in lockstep, and the leaf visit is shared (the lanes
+// split its entries and write the hits consecutively). The count comes from
+// kd_within_count as for the thread form.
 template <class A, int K, int G>
 __global__ __launch_bounds__(kBlock) void k_within_fill_group(DevTree<A, K> t, const A* __restrict__ queries,
                                                               const uint32_t* __restrict__ order, uint64_t m,
@@ -230,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_within_fill_group(DevTree<A, K> t, c
   const uint64_t o = offsets[qi], cap = offsets[qi + 1] - o;
   GroupWithinVis<O, G> v{radius, excl != 0, gl, 0, out_items + o, out_d + o, cap};
   Counters c;
-  group_traverse<A, K, G>(t, q, qw, v, c, gl);
+  traverse(t, q, qw, v, c);
   if (sorted && gl == 0) sort_segment(out_items + o, out_d + o, v.count < cap ? v.count : cap);
 }
 

@@ -344,6 +344,24 @@ template <class M> int mlaunch_within_count(const QueryArgs& a, M met, double ra
     auto dt = dev_tree<A, KK>(a.t);
     reset_counters(a.t, a.s);
     KernelTimer kt(a.t, a.s);
+    if constexpr (std::is_same<M, dev::SqEuclidean>::value) {
+      // G lanes per query as tuned (kd_tree_tune_within); one thread per query by default
+      auto group = [&](auto gc) -> int {
+        constexpr int G = decltype(gc)::value;
+        hipLaunchKernelGGL((dev::k_within_count_group<A, KK, G>), dim3(grid_for(a.m * G)), dim3(dev::kBlock),
+                           lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, O(radius),
+                           (a.flags & KD_EXCLUSIVE) ? 1 : 0, counts, a.t->counters);
+        KD_HIP(hipGetLastError());
+        return KD_OK;
+      };
+      switch (a.t->count_group) {
+        case 4: return group(std::integral_constant<int, 4>{});
+        case 8: return group(std::integral_constant<int, 8>{});
+        case 16: return group(std::integral_constant<int, 16>{});
+        case 32: return group(std::integral_constant<int, 32>{});
+        default: break;  // one thread per query
+      }
+    }
     hipLaunchKernelGGL((dev::k_within_count<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds_bytes_for(dt),
                        a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, O(radius),
                        (a.flags & KD_EXCLUSIVE) ? 1 : 0, counts, a.t->counters);
@@ -360,14 +378,24 @@ int mlaunch_within_fill(const QueryArgs& a, M met, double radius, const uint64_t
     auto dt = dev_tree<A, KK>(a.t);
     KernelTimer kt(a.t, a.s);
     if constexpr (std::is_same<M, dev::SqEuclidean>::value) {
-      if (a.dense) {  // 16 lanes per query, coalesced result writes
-        constexpr int G = 16;
+      // G lanes per query with coalesced result writes: by default for dense outputs (16
+      // lanes), or as tuned (kd_tree_tune_within)
+      const int g = a.t->fill_group ? a.t->fill_group : (a.dense ? 16 : 1);
+      auto group = [&](auto gc) -> int {
+        constexpr int G = decltype(gc)::value;
         hipLaunchKernelGGL((dev::k_within_fill_group<A, KK, G>), dim3(grid_for(a.m * G)), dim3(dev::kBlock),
                            lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, O(radius),
                            (a.flags & KD_EXCLUSIVE) ? 1 : 0, (a.flags & KD_UNSORTED) ? 0 : 1, offsets, items,
                            static_cast<O*>(dists));
         KD_HIP(hipGetLastError());
         return KD_OK;
+      };
+      switch (g) {
+        case 4: return group(std::integral_constant<int, 4>{});
+        case 8: return group(std::integral_constant<int, 8>{});
+        case 16: return group(std::integral_constant<int, 16>{});
+        case 32: return group(std::integral_constant<int, 32>{});
+        default: break;  // 1: one thread per query
       }
     }
     hipLaunchKernelGGL((dev::k_within_fill<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds_bytes_for(dt),

@@ -185,3 +185,24 @@ def test_screen_four_dims(ka, oracle_mod, cuda, kind):
     r = gt.nearest_n_raw(q, 16, flags=ka.KD_SCREEN_F32 | ka.KD_MORTON)
     _cmp((r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy()), O.nearest_n(ot, q, 16),
          f"{kind} device-built screened k=16")
+
+
+def test_morton_order_is_the_visit_order(ka, oracle_mod, cuda):
+    """kd_morton_order returns a permutation of the batch; answering the queries permuted by
+    it gives the permuted answers, and chunks of it answer like the whole batch."""
+    torch = cuda
+    rng = np.random.default_rng(31)
+    pts = rng.random((40_000, 3))
+    q = rng.random((20_001, 3))
+    t = ka.ImmutableKdTree.new_from_slice(pts, 32)
+    perm = t.morton_order(q).view(torch.int32).long()
+    ph = perm.cpu().numpy()
+    assert np.array_equal(np.sort(ph), np.arange(len(q)))
+    whole = t.nearest_n_raw(q, 8, flags=ka.KD_MORTON)
+    qd = torch.from_numpy(q).cuda()
+    for a, b in ((0, 7000), (7000, 20_001)):
+        part = t.nearest_n_raw(qd.index_select(0, perm[a:b]).contiguous(), 8)
+        assert torch.equal(part.items.view(torch.int32), whole.items.view(torch.int32)[perm[a:b]])
+        assert torch.equal(part.distances.view(torch.int64), whole.distances.view(torch.int64)[perm[a:b]])
+    one = t.morton_order(q[:1])
+    assert one.cpu().tolist() == [0]

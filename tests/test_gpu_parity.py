@@ -279,16 +279,24 @@ def test_within_dense_group_fill(ka, oracle_mod, cuda, dtype):
         oo, oi, od = ot.within(q, r2, excl, True)
         assert int(oo[-1]) >= 32 * len(q)
         for flags in (0, ka.KD_MORTON):
-            f = flags | (ka.KD_EXCLUSIVE if excl else 0)
-            r = t.within_raw(q, r2, f)
-            assert_same_bits(r.offsets.cpu().numpy().astype(np.uint64), oo, "offsets")
-            assert_same_bits(r.items.cpu().numpy(), oi, "items")
-            assert_same_bits(r.distances.cpu().numpy(), od, "dists")
-            ru = t.within_raw(q, r2, f | ka.KD_UNSORTED)
-            ui, ud = ru.items.cpu().numpy(), ru.distances.cpu().numpy()
-            for i in range(0, len(q), 37):
-                s, e = int(oo[i]), int(oo[i + 1])
-                assert sorted(zip(ud[s:e].tolist(), ui[s:e].tolist())) == list(zip(od[s:e].tolist(), oi[s:e].tolist()))
+            for count, fill in ((0, 0), (1, 1), (4, 4), (8, 8), (16, 16), (32, 32), (8, 1), (1, 16)):
+                t.tune_within(count, fill)  # every launch shape gives the same results
+                f = flags | (ka.KD_EXCLUSIVE if excl else 0)
+                r = t.within_raw(q, r2, f)
+                assert_same_bits(r.offsets.cpu().numpy().astype(np.uint64), oo, "offsets")
+                assert_same_bits(r.items.cpu().numpy(), oi, f"items shapes={count},{fill}")
+                assert_same_bits(r.distances.cpu().numpy(), od, f"dists shapes={count},{fill}")
+                ru = t.within_raw(q, r2, f | ka.KD_UNSORTED)
+                ui, ud = ru.items.cpu().numpy(), ru.distances.cpu().numpy()
+                qid = np.repeat(np.arange(len(q)), np.diff(oo.astype(np.int64)))
+                key = np.lexsort((ui, ud, qid))
+                assert_same_bits(ui[key], oi, f"unsorted items as multisets shapes={count},{fill}")
+                assert_same_bits(ud[key], od, f"unsorted dists as multisets shapes={count},{fill}")
+            t.tune_within(0, 0)
+            chosen = t.autotune_within(q, r2, flags)
+            r = t.within_raw(q, r2, flags | (ka.KD_EXCLUSIVE if excl else 0))
+            assert_same_bits(r.items.cpu().numpy(), oi, f"items autotuned {chosen}")
+            t.tune_within(0, 0)
 
 
 @pytest.mark.parametrize("r2,dense", [(0.006, True), (0.0004, False)], ids=["dense", "sparse"])
