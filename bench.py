@@ -261,7 +261,7 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel=("k_nearest_n_screen" if head == "screened" else "k_nearest_n"),
+                    kernel=("k_nearest_n_screen32" if head == "screened" else "k_nearest_n"),
                     kernel_ms=round(results[head]["kernel_ms"], 3),
                     algo_bytes_per_launch=int(algo_bytes))
 

@@ -276,6 +276,11 @@ int kd_periodic_within_fill(kd_tree* tree, const void* queries, uint64_t m, cons
  * read, entries re-measured in FP64 by KD_SCREEN_F32}. The first three equal what Kiddo's
  * DFS visits for the same queries (its leaf-visit hook: query/nearest_one.rs:31-32). */
 int kd_query_stats(kd_tree* tree, uint64_t* out4);
+/* Queries of the last call on this tree that the FP32-screened k-NN kernel (k <= 16) could
+ * not settle from its candidate list (more entries within the certified rounding band of the
+ * k-th distance than the list holds, e.g. many exact ties) and answered again with the plain
+ * FP64 kernel (no Kiddo counterpart: measurement support). */
+int kd_query_redos(kd_tree* tree, uint64_t* out);
 
 /* Timing of the main query kernel of each call with HIP events recorded on the call's
  * stream (no Kiddo counterpart: measurement support). kd_last_kernel_ms waits for the

@@ -119,6 +119,7 @@ SIGNATURES = {
     "kd_periodic_within_count": ([_vp, _vp, _u64, _vp, _f64, _u32, _vp, _vp], C.c_int),
     "kd_periodic_within_fill": ([_vp, _vp, _u64, _vp, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
     "kd_query_stats": ([_vp, _vp], C.c_int),
+    "kd_query_redos": ([_vp, _vp], C.c_int),
     "kd_forest_create": ([_vp, _u32, _u32, C.POINTER(_vp)], C.c_int),
     "kd_forest_destroy": ([_vp], C.c_int),
     "kd_forest_plan": ([_vp, _vp, _vp, _u32, _u32, _vp], C.c_int),
@@ -409,6 +410,13 @@ class ImmutableKdTree:
         out = np.zeros(4, np.uint64)
         _check(lib().kd_query_stats(self._h, _np_ptr(out)), "kd_query_stats")
         return dict(zip(("leaves", "points", "stems", "rescored"), (int(x) for x in out)))
+
+    def query_redos(self) -> int:
+        """Queries of the last call that the FP32-screened k-NN kernel handed to its exact
+        FP64 redo pass (candidate list overflowed, e.g. by many exact ties)."""
+        out = np.zeros(1, np.uint64)
+        _check(lib().kd_query_redos(self._h, _np_ptr(out)), "kd_query_redos")
+        return int(out[0])
 
     def tune(self, knn_variant: int = 0, group_size: int = 0, lds_levels: int = -1, waves: int = 0):
         """Kernel form for k-NN (0 auto, 1 thread/query, 2 group/query), LDS stem levels and

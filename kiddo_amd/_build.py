@@ -29,7 +29,7 @@ HIP_FLAGS = [
 ]
 
 
-SOURCES = ["api.hip", "k_knn.hip", "k_families.hip", "k_metric_manhattan.hip", "k_metric_chebyshev.hip",
+SOURCES = ["api.hip", "k_knn.hip", "k_knn_f64.hip", "k_families.hip", "k_metric_manhattan.hip", "k_metric_chebyshev.hip",
            "k_metric_minkowski.hip", "k_build.hip", "k_forest.hip", "server.hip", "host_build.cpp"]
 OBJ_DIR = os.path.join(HERE, "_build_obj")
 

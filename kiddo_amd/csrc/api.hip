@@ -62,7 +62,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 uint32_t grid_for(uint64_t threads) { return uint32_t((threads + dev::kBlock - 1) / dev::kBlock); }
 
 void reset_counters(kd_tree* t, hipStream_t s) {
-  hipLaunchKernelGGL(dev::k_zero_u64, dim3(1), dim3(dev::kBlock), 0, s, t->counters, 4 * dev::kStatSlots);
+  hipLaunchKernelGGL(dev::k_zero_u64, dim3(1), dim3(dev::kBlock), 0, s, t->counters, dev::kCounterWords);
 }
 
 uint32_t lds_stems_for(const kd_tree* t, size_t a_size) {
@@ -250,7 +250,7 @@ template <class A, int K> int create_typed(const kd_tree_desc* desc, uint32_t fl
   KD_TRY(dmalloc(&t->items, std::max<uint64_t>(n, 1) * sizeof(uint32_t), tot));
   KD_TRY(dmalloc(&t->d_max_abs, 4 * sizeof(double), tot));
   KD_TRY(dmalloc(&t->d_bbox, 8 * sizeof(double), tot));
-  KD_TRY(dmalloc(&t->counters, 4 * dev::kStatSlots * sizeof(unsigned long long), tot));
+  KD_TRY(dmalloc(&t->counters, dev::kCounterWords * sizeof(unsigned long long), tot));
   if (!shadow.empty()) KD_TRY(dmalloc(&t->shadow, shadow.size() * sizeof(float4), tot));
   double bb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int d = 0; d < K; ++d) {
@@ -271,7 +271,7 @@ template <class A, int K> int create_typed(const kd_tree_desc* desc, uint32_t fl
   KD_TRY(up(t->d_max_abs, mabs, 4 * sizeof(double)));
   KD_TRY(up(t->d_bbox, bb, 8 * sizeof(double)));
   if (!shadow.empty()) KD_TRY(up(t->shadow, shadow.data(), shadow.size() * sizeof(float4)));
-  if (hipMemset(t->counters, 0, 4 * dev::kStatSlots * sizeof(unsigned long long)) != hipSuccess) {
+  if (hipMemset(t->counters, 0, dev::kCounterWords * sizeof(unsigned long long)) != hipSuccess) {
     free_tree(t);
     return set_error(KD_ERR_DEVICE, "hipMemset failed");
   }
@@ -1233,6 +1233,16 @@ int kd_query_stats(kd_tree* t, uint64_t* out4) {
   KD_HIP(hipMemcpy(h.data(), t->counters, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost));
   for (int j = 0; j < 4; ++j) out4[j] = 0;
   for (size_t i = 0; i < h.size(); ++i) out4[i % 4] += h[i];
+  return KD_OK;
+}
+
+int kd_query_redos(kd_tree* t, uint64_t* out) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (!out) return kd::set_error(KD_ERR_INVALID, "out is null");
+  unsigned long long h = 0;
+  KD_HIP(hipDeviceSynchronize());
+  KD_HIP(hipMemcpy(&h, t->counters + kd::dev::kRedoWord, sizeof(h), hipMemcpyDeviceToHost));
+  *out = h;
   return KD_OK;
 }
 

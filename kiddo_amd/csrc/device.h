@@ -79,6 +79,10 @@ template <class T, int K> struct Small {
 constexpr int kMaxLevels = 32;
 constexpr int kBlock = 256;  // threads per block of every query kernel
 constexpr int kStatSlots = 256;  // query-statistics counters: groups of 4, summed on read
+// The counter array holds the statistics groups, then one word that counts the queries the
+// screened k-NN kernel hands to its exact redo pass (k_nearest_n_redo), then padding.
+constexpr int kRedoWord = 4 * kStatSlots;
+constexpr int kCounterWords = 4 * kStatSlots + 4;
 // The FP32 shadow array holds this many (zero) entries past the last point, so that batched
 // loads may run past the end of a leaf without clamping their indices.
 constexpr uint32_t kShadowPad = 8;
@@ -1126,6 +1130,326 @@ template <int MAXK> struct ScreenKnnVis {
     // The bound only tightens, so a threshold from the start of the visit stays valid for
     // the mask; refresh it once for the next visit.
     thr = screen_threshold(v.bound(), eps_k);
+  }
+};
+
+// Certified upper bound on the FP64 distance of an entry whose FP32 screen distance is s:
+// the FP32 sum of K <= 4 non-negative squares is at least ||delta||^2 (1-u)^K - K 2^-150
+// (a product that underflows into the subnormal range loses up to 2^-150 absolute, a sum
+// with a subnormal result is exact; 1e-44 > 4 * 2^-150, and 1 / (1-u)^4 < 1 + 5u), each FP32
+// coordinate difference is within eps of the exact one, so the exact distance is at most
+// (sqrt((s + 1e-44)(1 + 5u)) + sqrt(K) eps)^2, and the FP64 evaluation is within (1 + 1e-15)
+// of it. The FP64 square root is within a few ulps, which the (1 + 1e-15) factors cover.
+__device__ __forceinline__ double fp64_cap(float s, double eps_k) {
+  if (!(s < __builtin_huge_valf())) return __builtin_huge_val();
+  const double r = sqrt((double(s) + 1e-44) * (1.0 + 5.0 * 0x1p-24)) * (1.0 + 1e-15) + eps_k;
+  return r * r * (1.0 + 2e-15);
+}
+
+// Largest FP32 screen distance that certifies an FP64 distance within the radius R:
+// fp64_cap(s) <= R for every s <= the returned value (inverse of fp64_cap, rounded down and
+// checked). +inf for an infinite radius, -1 when no screen distance certifies it.
+__device__ __forceinline__ float certify_limit(double R, double eps_k) {
+  if (!(R < __builtin_huge_val())) return __builtin_huge_valf();
+  const double r = sqrt(R / (1.0 + 4e-15)) * (1.0 - 1e-15) - eps_k;
+  if (!(r > 0.0)) return -1.0f;
+  const double s = (r / (1.0 + 2e-15)) * (r / (1.0 + 2e-15)) / (1.0 + 6.0 * 0x1p-24) - 1e-44;
+  if (!(s > 0.0)) return -1.0f;
+  float f = (float)s;
+  if ((double)f > s) f = __uint_as_float(__float_as_uint(f) - 1u);
+  while (f > 0.0f && !(fp64_cap(f, eps_k) <= R)) f = __uint_as_float(__float_as_uint(f) - 1u);
+  return f;
+}
+
+// Per-query constants of the FP32 bound refresh (Screen32Vis::refresh).
+// With A = sqrt(L + 1e-44) for an FP32 screen distance L and c = 1 + 2^-16:
+//   * fp64_cap(L) <= (A (1 + 2.6u) + eps_k)^2 (1 + 2e-15) <= fl32(fl32(c fl32(sqrt(L + 2^-146)) + e1)^2 c),
+//     e1 = eps_k (1 + 2^-16) rounded up: every FP32 operation loses at most a factor (1 - u),
+//     and c's 256u margin covers the square root's and the other operations' rounding;
+//   * screen_threshold(fp64_cap(L)) <= (A (1 + 67u) + 2.0001 eps_k)^2 (1 + 6u) + 1e-44 (the
+//     threshold takes a square root inflated by 2^-19 = 32u and rounds up),
+//     <= fl32(fl32(c s' + e2)^2 c) + 2^-146 with e2 = 2 eps_k (1 + 2^-16) rounded up
+//     (2^-146 > 1e-44; for results in the subnormal range the addition is exact, above it the
+//     factor c covers it).
+// So the FP32 forms bound the FP64 ones from above and prune and screen no more than they
+// do: the refresh after each visit costs a handful of FP32 operations instead of two square
+// roots in FP64.
+__device__ __forceinline__ float round_up_f32(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = next_up(f);
+  return f;
+}
+
+// FP32 collector of the screened k-NN (k <= 16). The traversal runs on FP32 distances only;
+// FP64 is used to rescore the final candidates and for the returned distances.
+//   * top[] holds the MAXK smallest FP32 distances of the entries seen so far whose FP64
+//     distance is certified within the radius (fp64_cap(s) <= radius), distances only.
+//     With L = top[k-1], k entries have an FP64 distance of at most fp64_cap(L) <= U (the
+//     FP32 form above), so the k-th FP64 distance of the answer is at most U: the traversal
+//     prunes by bound = min(U, radius), like Kiddo's by the k-th distance (an entry of the
+//     answer has a distance of at most the bound of every moment, so it is never pruned).
+//   * The pre-scan's block (the first visit) is in top[] before the traversal starts; its
+//     entries are screened again at the end.
+//   * In later visits, an entry whose FP32 distance passes thr (every entry whose FP64
+//     distance can be at most the bound passes) is appended to a list of candidates in LDS
+//     (kCand entry indices per lane). When the list is full it is compacted against the
+//     current threshold: an entry of the answer passes every threshold, so none is dropped.
+//     A list that is still full after compaction cannot prove it holds the whole answer:
+//     the query is marked for the exact redo pass (k_nearest_n_redo) and its traversal ends
+//     (every subtree pruned).
+//   * At the end the block's entries and the listed ones that pass the final threshold are
+//     measured in FP64 and the k smallest in (distance, item) order are kept (Batcher's
+//     network for the first MAXK, insertions for the rest): the plain kernel's answer.
+// A survivor of the screen costs a med3 per slot of top[] and one LDS store, instead of an
+// FP64 measurement and an insertion into an FP64 (distance, item) list.
+template <int MAXK> struct Screen32Vis {
+  static_assert(MAXK <= 16, "the FP32 collector is for k <= 16");
+  static constexpr uint32_t kCand = MAXK == 1 ? 8u : uint32_t(MAXK) + 8u;
+  static constexpr int kWords = MAXK >= 16 ? 2 : 1;
+  static constexpr uint32_t kMaskN = 64u * kWords;
+  static constexpr float kC = 1.0f + 0x1p-16f;
+  float top[MAXK];  // ascending; see init
+  int k;
+  double radius;
+  bool excl;
+  bool pre;       // the first visit is the pre-scan block, whose distances are in top already
+  bool ovf;       // the candidate list overflowed: the query goes to the redo pass
+  float qf[4];    // the query in FP32
+  double eps_k;   // sqrt(K) * per-query coordinate-difference error bound
+  float e1, e2;   // eps_k and 2 eps_k, inflated and rounded up (refresh)
+  float rlow;     // certify_limit(radius)
+  float thr_r;    // screen_threshold(radius)
+  float thr;      // FP32 threshold of the current bound
+  double bnd;     // min(U, radius); -inf after an overflow
+  uint32_t nc;    // candidates in the list
+  uint32_t visits, rescored;
+  uint32_t* cand;  // LDS: kCand x kBlock entry indices, slot j of lane l at j * kBlock + l
+
+  // after qf and eps_k are set
+  __device__ void init(int k_, double r, bool ex, uint32_t* cand_lds) {
+    // the first MAXK - k slots hold -inf and never leave the front, so the last slot is
+    // always the k-th smallest distance (a constant index: a run-time one puts top[] in
+    // scratch memory)
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) top[j] = j < MAXK - k_ ? -__builtin_huge_valf() : __builtin_huge_valf();
+    k = k_;
+    radius = r;
+    excl = ex;
+    pre = false;
+    ovf = false;
+    nc = 0;
+    visits = 0;
+    rescored = 0;
+    cand = cand_lds;
+    e1 = round_up_f32(eps_k * (1.0 + 0x1p-16));
+    e2 = round_up_f32(2.0 * eps_k * (1.0 + 0x1p-16));
+    rlow = certify_limit(r, eps_k);
+    thr_r = screen_threshold(r, eps_k);
+    bnd = r;
+    thr = thr_r;
+  }
+  __device__ double bound() const { return bnd; }
+  __device__ bool prune(double rd) const { return prune_at(rd, bnd); }
+  __device__ bool prune_at(double rd, double b) const { return ovf || rd > b || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return bnd == __builtin_huge_val(); }
+  // bound and threshold from top[] (called when top[] may have changed)
+  __device__ __forceinline__ void refresh() {
+    const float L = top[MAXK - 1];
+    float U32 = __builtin_huge_valf(), t32 = __builtin_huge_valf();
+    if (L < __builtin_huge_valf()) {
+      const float s = sqrtf(L + 0x1p-146f) * kC;
+      const float u = s + e1, t = s + e2;
+      U32 = u * u * kC;
+      t32 = t * t * kC + 0x1p-146f;
+    }
+    const double U = double(U32);
+    bnd = ovf ? -__builtin_huge_val() : (U < radius ? U : radius);
+    thr = t32 < thr_r ? t32 : thr_r;
+  }
+  // sorted insertion of a distance (+inf changes nothing)
+  __device__ __forceinline__ void insert32(float c) {
+#pragma unroll
+    for (int j = MAXK - 1; j > 0; --j) top[j] = med3(top[j - 1], c, top[j]);
+    top[0] = fminf(top[0], c);
+  }
+  template <int K>
+  __device__ __forceinline__ float screen_dist(const DevTree<double, K>& t, uint32_t p) const {
+    return shadow_sq<K>(t.shadow[p], qf);
+  }
+  template <int K>
+  __device__ __forceinline__ uint32_t next_tail(const DevTree<double, K>& t, uint32_t s, uint32_t next,
+                                                uint32_t L) const {
+    while (next < L && screen_dist(t, s + next) > thr) ++next;
+    return next;
+  }
+  // keeps the candidates that pass the current threshold, in order
+  template <int K> __device__ __forceinline__ void compact(const DevTree<double, K>& t) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t w = 0;
+    for (uint32_t j = 0; j < nc; ++j) {
+      const uint32_t p = cand[j * kBlock + lane];
+      const bool keep = !(screen_dist(t, p) > thr);
+      cand[w * kBlock + lane] = p;
+      w += keep ? 1u : 0u;
+    }
+    nc = w;
+  }
+  template <int K> __device__ __forceinline__ void append(const DevTree<double, K>& t, uint32_t p) {
+    if (nc == kCand) {
+      refresh();
+      compact(t);
+      if (nc == kCand) {
+        ovf = true;
+        bnd = -__builtin_huge_val();
+        return;
+      }
+    }
+    cand[nc * kBlock + threadIdx.x] = p;
+    ++nc;
+  }
+  // marks in mk the entries among [s, s + nm) whose FP32 distance passes thr (shadow loads
+  // issued 4 at a time from one address, up to 3 past the range: the shadow's padding)
+  template <int K>
+  __device__ __forceinline__ void screen_into(const DevTree<double, K>& t, uint32_t s, uint32_t nm,
+                                              EntryMask<kWords>& mk) const {
+    constexpr uint32_t kBatch = 4;
+#pragma unroll
+    for (int w = 0; w < kWords; ++w) {
+      const uint32_t lo = 64u * uint32_t(w), hi = nm < lo + 64u ? nm : lo + 64u;
+      uint64_t word = 0;
+      for (uint32_t j0 = lo; j0 < hi; j0 += kBatch) {
+        const float4* __restrict__ sp = t.shadow + s + j0;
+        float4 sh[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) sh[u] = sp[u];
+        uint32_t bits = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+          const float acc = shadow_sq<K>(sh[u], qf);
+          bits |= ((j0 + u < nm) && !(acc > thr) ? 1u : 0u) << u;
+        }
+        word |= uint64_t(bits) << (j0 - lo);
+      }
+      mk.m[w] |= word;
+    }
+  }
+  // A visit after the pre-scan block: the screen marks the survivors among the first kMaskN
+  // entries against the threshold of the start of the visit, entries past the mask are
+  // screened one at a time; each survivor's FP32 distance goes into top[] and its index into
+  // the candidate list. The bound and threshold are refreshed at the end of the visit (and
+  // after each entry past the mask, for long leaves of equal coordinates).
+  template <int K, class QW>
+  __device__ __forceinline__ void leaf(const DevTree<double, K>& t, const QW&, uint32_t leaf, uint32_t nl) {
+    const bool block = visits == 0u && pre;
+    visits += 1u;
+    if (nl == 0u || ovf || block) return;
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
+    const uint32_t L = e - s, nm = L < kMaskN ? L : kMaskN;
+    const bool screen = thr < __builtin_huge_valf();
+    EntryMask<kWords> mk;
+    mk.clear();
+    if (!screen) mk.set_first(nm);
+    screen_into(t, s, screen ? nm : 0u, mk);
+    uint32_t next = nm;  // first entry not covered by the mask
+    uint32_t p = 0;
+    bool have = true;
+    auto select = [&]() {
+      have = true;
+      if (mk.any()) {
+        p = s + mk.pop();
+      } else {
+        next = next_tail(t, s, next, L);
+        if (next < L) {
+          p = s + next;
+          ++next;
+        } else {
+          have = false;
+        }
+      }
+    };
+    select();
+    float sn = have ? screen_dist(t, p) : 0.0f;
+    while (have) {
+      const uint32_t pc = p;
+      const float sc = sn;
+      select();
+      sn = screen_dist(t, have ? p : pc);  // the next survivor's load, a step ahead
+      insert32(sc <= rlow ? sc : __builtin_huge_valf());
+      append(t, pc);
+      have = have && !ovf;
+      if (pc - s + 1u >= kMaskN) refresh();
+    }
+    refresh();
+  }
+  // The answer: the pre-scan block's entries [bb, be) and the listed candidates that pass
+  // the final threshold, measured in FP64, in (distance, item) order; ok entries are those
+  // within the radius.
+  template <int K, class QW>
+  __device__ __forceinline__ void finish(const DevTree<double, K>& t, const QW& qw, uint32_t bb, uint32_t be,
+                                         TopK<double, MAXK>& res, int& cnt) {
+    compact(t);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t L = be - bb, nm = L < kMaskN ? L : kMaskN;
+    EntryMask<kWords> mk;
+    mk.clear();
+    const bool screen = thr < __builtin_huge_valf();
+    if (!screen) mk.set_first(nm);
+    screen_into(t, bb, screen ? nm : 0u, mk);
+    uint32_t next = nm, li = 0, taken = 0;
+    // the next candidate: the block's mask, the block past the mask, then the list
+    auto next_cand = [&](uint32_t& p) -> bool {
+      if (mk.any()) {
+        p = bb + mk.pop();
+        return true;
+      }
+      next = next_tail(t, bb, next, L);
+      if (next < L) {
+        p = bb + next;
+        ++next;
+        return true;
+      }
+      if (li < nc) {
+        p = cand[li * kBlock + lane];
+        ++li;
+        return true;
+      }
+      return false;
+    };
+    // the first MAXK together (an empty slot reads entry bb, in the tree), so that all their
+    // loads are in flight at once, sorted by Batcher's network
+    uint32_t pidx[MAXK];
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) {
+      uint32_t p = bb;
+      taken += next_cand(p) ? 1u : 0u;
+      pidx[j] = p;
+    }
+    int nok = 0;
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j) {
+      const double dd = point_dist(t, pidx[j], qw);
+      const uint32_t it = t.items[pidx[j]];
+      const bool ok = (uint32_t(j) < taken) & ((dd < radius) | (!excl & (dd == radius)));
+      // the item is masked rather than selected (a select of a loaded value becomes a
+      // branch around its load); the empty asm keeps the mask opaque
+      uint32_t drop = ok ? 0u : 0xFFFFFFFFu;
+      asm("" : "+v"(drop));
+      res.d[j] = ok ? dd : __builtin_huge_val();
+      res.it[j] = it | drop;
+      nok += ok ? 1 : 0;
+    }
+    res.sort();
+    uint32_t p = bb;
+    while (next_cand(p)) {
+      ++taken;
+      const double dd = point_dist(t, p, qw);
+      const uint32_t it = t.items[p];
+      const bool ok = (dd < radius) | (!excl & (dd == radius));
+      nok += ok ? 1 : 0;
+      res.insert_if(ok && TopK<double, MAXK>::less(dd, it, res.d[MAXK - 1], res.it[MAXK - 1]), dd, it);
+    }
+    rescored = taken;
+    cnt = nok < k ? nok : k;
   }
 };
 

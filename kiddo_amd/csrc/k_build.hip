@@ -381,7 +381,7 @@ int build_typed(kd_tree* t, const A* pts, const uint32_t* items, uint64_t n, int
   KD_B(dev_alloc(reinterpret_cast<void**>(&t->items), std::max<uint64_t>(n, 1) * 4));
   KD_B(dev_alloc(reinterpret_cast<void**>(&t->d_max_abs), 4 * sizeof(double)));
   KD_B(dev_alloc(reinterpret_cast<void**>(&t->d_bbox), 8 * sizeof(double)));
-  KD_B(dev_alloc(reinterpret_cast<void**>(&t->counters), 4 * dev::kStatSlots * sizeof(unsigned long long)));
+  KD_B(dev_alloc(reinterpret_cast<void**>(&t->counters), dev::kCounterWords * sizeof(unsigned long long)));
   if (flags & KD_TREE_SHADOW_F32) {
     KD_B(dev_alloc(reinterpret_cast<void**>(&t->shadow), (n + dev::kShadowPad) * 16));
     KD_HIP(hipMemsetAsync(t->shadow + n, 0, dev::kShadowPad * 16, s));  // the padding
@@ -401,8 +401,8 @@ int build_typed(kd_tree* t, const A* pts, const uint32_t* items, uint64_t n, int
   }
   KD_HIP(hipMemcpyAsync(t->d_max_abs, mabs, 4 * sizeof(double), hipMemcpyHostToDevice, s));
   KD_HIP(hipMemcpyAsync(t->d_bbox, bb, 8 * sizeof(double), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(dev::k_fill<unsigned long long>, dim3(grid_for(4 * dev::kStatSlots)), dim3(dev::kBlock), 0, s,
-                     t->counters, 4 * dev::kStatSlots, 0ull);
+  hipLaunchKernelGGL(dev::k_fill<unsigned long long>, dim3(grid_for(dev::kCounterWords)), dim3(dev::kBlock), 0, s,
+                     t->counters, dev::kCounterWords, 0ull);
   KD_HIP(hipGetLastError());
   KD_HIP(hipStreamSynchronize(s));
 #undef KD_B

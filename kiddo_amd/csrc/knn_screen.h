@@ -1,0 +1,140 @@
+// knn_screen.h — the FP32-screened k-NN kernel for f64 trees with k <= 16 (FP32 collector,
+// device.h Screen32Vis) and the exact redo pass for the queries whose candidate list
+// overflowed (kiddo query/nearest_n.rs:21-38, nearest_n_within.rs:127-179, :441-479).
+#pragma once
+
+#include "families.h"
+
+namespace kd {
+namespace dev {
+
+// Pre-scan of the FP32 collector: the query descends to its leaf (the traversal's rule: a
+// sentinel stem sends it left) and takes the smallest block of 2^j sibling leaves holding 4k
+// entries (64 at least from k = 8 up; leaves under one stem are contiguous), whose FP32
+// distances of certified in-radius entries fill top[] (all lanes of a wave do this in step).
+// The block is then the traversal's first visit. `up` receives the block's height in levels,
+// [b, e) its entries. The tree must not be empty.
+template <int K, int MAXK>
+__device__ __forceinline__ void prescan_top(const DevTree<double, K>& t, const Small<double, K>& q, Screen32Vis<MAXK>& v,
+                            int& up_out, uint32_t& b_out, uint32_t& e_out) {
+  const int D = t.nlev;
+  uint32_t idx = 1;
+  int dim = 0, level = 0;
+  auto step = [&](const double pivot) {
+    idx = (idx << 1) | (((pivot < Ax<double>::maxv()) & (q[dim] >= pivot)) ? 1u : 0u);
+    dim = (dim + 1 == K) ? 0 : dim + 1;
+    ++level;
+  };
+  const int lds_levels = t.lds_n ? 31 - __clz(t.lds_n) : 0;
+  const int mid = D < lds_levels ? D : lds_levels;
+  while (level < mid) step(t.lds[idx]);
+  while (level < D) step(t.stems[idx]);
+  const uint32_t leaf = idx - (1u << D);
+  const uint32_t want = v.k >= 8 && 4 * v.k < 64 ? 64u : uint32_t(4 * v.k);
+  uint32_t b = 0, e = 0;
+  int up = 0;
+  for (;; ++up) {
+    const uint32_t first = (leaf >> up) << up;
+    b = t.leaf_start[first];
+    e = t.leaf_start[first + (1u << up)];
+    if (e - b >= want || up >= D) break;
+  }
+  constexpr uint32_t kBatch = 8;
+  for (uint32_t p0 = b; p0 < e; p0 += kBatch) {
+    float d32[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) {
+      const float acc = shadow_sq<K>(t.shadow[p0 + u], v.qf);  // up to 7 past e: the shadow's padding
+      d32[u] = (p0 + u < e) & (acc <= v.rlow) ? acc : __builtin_huge_valf();
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) v.insert32(d32[u]);
+  }
+  up_out = up;
+  b_out = b;
+  e_out = e;
+}
+
+// nearest_n on f64 trees with k <= 16, FP32-screened with the FP32 collector
+// (Screen32Vis). Queries whose candidate list overflowed get count kRedo and are answered
+// by k_nearest_n_redo, launched after this kernel on the same stream.
+constexpr uint32_t kRedo = 0xFFFFFFFFu;
+
+template <int K, int MAXK, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_nearest_n_screen32(
+    DevTree<double, K> t, const double* __restrict__ queries, const uint32_t* __restrict__ order,
+    uint64_t m, int k, double radius, int excl, const double* __restrict__ max_abs,
+    uint32_t* __restrict__ out_items, double* __restrict__ out_d, uint32_t* __restrict__ out_counts,
+    unsigned long long* __restrict__ stats) {
+  using A = double;
+  __shared__ uint32_t cand[Screen32Vis<MAXK>::kCand * kBlock];
+  KD_QUERY_PROLOGUE
+  if (active) {
+    Screen32Vis<MAXK> v;
+    screen_setup<K>(q, max_abs, v.qf, v.eps_k);
+    v.init(k, radius, excl != 0, cand);
+    int up = 0;
+    uint32_t bb = 0, be = 0;
+    if (t.n != 0) {
+      prescan_top<K, MAXK>(t, q, v, up, bb, be);
+      v.pre = true;
+    }
+    v.refresh();
+    traverse(t, q, qw, v, cnt, up);
+    uint32_t* ri = out_items + qi * uint64_t(k);
+    double* rdd = out_d + qi * uint64_t(k);
+    if (v.ovf) {
+      out_counts[qi] = kRedo;
+      atomicAdd(stats + kRedoWord, 1ull);
+    } else {
+      TopK<double, MAXK> res;
+      int rc = 0;
+      v.finish(t, qw, bb, be, res, rc);
+#pragma unroll
+      for (int j = 0; j < MAXK; ++j)
+        if (j < k) {
+          ri[j] = j < rc ? res.it[j] : 0xFFFFFFFFu;
+          rdd[j] = j < rc ? res.d[j] : __builtin_huge_val();
+        }
+      out_counts[qi] = uint32_t(rc);
+    }
+    cnt.rescored = v.rescored;
+  }
+  flush_counters(cnt, stats);
+}
+
+// Exact redo of the queries k_nearest_n_screen32 marked (count kRedo): the plain FP64
+// traversal and collector, one thread per marked query. Returns at once when none was
+// marked; otherwise the grid strides over the counts in input order.
+template <int K, int MAXK>
+__global__ __launch_bounds__(kBlock) void k_nearest_n_redo(DevTree<double, K> t, const double* __restrict__ queries,
+                                                           uint64_t m, int k, double radius, int excl,
+                                                           uint32_t* __restrict__ out_items,
+                                                           double* __restrict__ out_d,
+                                                           uint32_t* __restrict__ out_counts,
+                                                           unsigned long long* __restrict__ stats) {
+  if (stats[kRedoWord] == 0ull) return;
+  Counters cnt;
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride) {
+    if (out_counts[i] != kRedo) continue;
+    Small<double, K> q, qw;
+    load_query<double, K>(queries, i, q, qw);
+    KnnVis<double, MAXK> v;
+    v.init(k, radius, excl != 0);
+    traverse(t, q, qw, v, cnt);
+    uint32_t* ri = out_items + i * uint64_t(k);
+    double* rdd = out_d + i * uint64_t(k);
+#pragma unroll
+    for (int j = 0; j < MAXK; ++j)
+      if (j < k) {
+        ri[j] = j < v.cnt ? v.top.it[j] : 0xFFFFFFFFu;
+        rdd[j] = j < v.cnt ? v.top.d[j] : __builtin_huge_val();
+      }
+    out_counts[i] = uint32_t(v.cnt);
+  }
+  flush_counters(cnt, stats);
+}
+
+}  // namespace dev
+}  // namespace kd

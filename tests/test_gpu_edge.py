@@ -179,7 +179,13 @@ def test_screen_four_dims(ka, oracle_mod, cuda, kind):
     t.tune(0, 0, -1)
     t.nearest_n_raw(q, 16, flags=ka.KD_SCREEN_F32)
     st = t.query_stats()
-    assert 0 < st["rescored"] <= st["points"], st
+    redos = t.query_redos()
+    assert st["rescored"] <= st["points"], st
+    # coordinates near 1e-22 have FP32 squares near the subnormal range, where the certified
+    # band (an absolute 1e-44 term) holds every entry: those queries go to the exact redo pass
+    assert st["rescored"] > 0 or redos > 0, (st, redos)
+    if kind == "uniform":
+        assert st["rescored"] > 0 and redos == 0, (st, redos)
     # built on the GPU: the same shadow layout
     gt = ka.ImmutableKdTree.build_on_device(pts, 32, shadow_f32=True)
     r = gt.nearest_n_raw(q, 16, flags=ka.KD_SCREEN_F32 | ka.KD_MORTON)

@@ -357,3 +357,39 @@ def test_screened_edge_cases(ka, oracle_mod, cuda, name, make, B):
                 got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
                 for a, b, nm in zip(got, ref, ("items", "dists", "counts")):
                     assert_same_bits(a, b, f"{name} k={k} r={radius} excl={excl} flags={f} {nm}")
+
+
+def test_screened_redo_pass(ka, oracle_mod, cuda):
+    """The FP32 collector keeps the candidates found after the pre-scan block in a list of
+    k + 8 entries. On a 7 x 7 x 7 grid holding ~117 exact copies of each point, a query on a
+    grid point ties with copies spread over many leaves: its list overflows and the exact
+    FP64 redo pass answers it. 300 exact copies of a point mostly share the query's pre-scan
+    block, whose ties are resolved in the kernel's final pass. Both against the oracle; the
+    redo count is reported by kd_query_redos."""
+    rng = np.random.default_rng(7)
+    grid = rng.integers(0, 7, (40_000, 3)).astype(np.float64)
+    hot = np.concatenate([rng.random((30_000, 3)), np.repeat(rng.random((3, 3)), 300, axis=0)])
+    for name, pts in (("grid", grid), ("hot", hot)):
+        q = rng.random((4000, 3)) * float(pts.max())
+        q[:1000] = pts[rng.integers(0, len(pts), 1000)]
+        t = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
+        O = OracleEngine(oracle_mod)
+        ot = O.build(pts, 32)
+        for k in (1, 4, 8, 16):
+            ref = O.nearest_n(ot, q, k)
+            for flags in (ka.KD_SCREEN_F32, ka.KD_SCREEN_F32 | ka.KD_MORTON):
+                _announce("screened redo", name, k, flags)
+                r = t.nearest_n_raw(q, k, math.inf, flags)
+                got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
+                for a, b, nm in zip(got, ref, ("items", "dists", "counts")):
+                    assert_same_bits(a, b, f"redo {name} k={k} flags={flags} {nm}")
+                redos = t.query_redos()
+                if name == "grid":  # every query ties with a whole stack of copies
+                    assert 0 < redos <= len(q), f"grid k={k}: {redos} queries redone"
+                else:
+                    assert redos <= 1100, (name, k, redos)
+    # no ties: nothing is redone
+    u = rng.random((20_000, 3))
+    tu = ka.ImmutableKdTree.new_from_slice(u, 32, shadow_f32=True)
+    tu.nearest_n_raw(rng.random((4000, 3)), 16, math.inf, ka.KD_SCREEN_F32 | ka.KD_MORTON)
+    assert tu.query_redos() == 0
