@@ -279,7 +279,8 @@ int kd_query_stats(kd_tree* tree, uint64_t* out4);
 /* Queries of the last call on this tree that the FP32-screened k-NN kernel (k <= 16) could
  * not settle from its candidate list (more entries within the certified rounding band of the
  * k-th distance than the list holds, e.g. many exact ties) and answered again with the plain
- * FP64 kernel (no Kiddo counterpart: measurement support). */
+ * FP64 kernel (no Kiddo counterpart: measurement support). Like kd_query_stats, a count that
+ * calls overlapping on one tree share; results never depend on it. */
 int kd_query_redos(kd_tree* tree, uint64_t* out);
 
 /* Timing of the main query kernel of each call with HIP events recorded on the call's

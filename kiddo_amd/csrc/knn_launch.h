@@ -204,8 +204,7 @@ int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, v
                          dim3(dev::kBlock), lds, a.s, dt, q, a.order, a.m, k, radius, excl, t->d_max_abs, items,
                          static_cast<double*>(dists), counts, t->counters);
       KD_HIP(hipGetLastError());
-      // the queries whose candidate list overflowed, answered exactly (returns at once when
-      // there are none); global stems only
+      // the queries whose candidate list overflowed, answered exactly; global stems only
       auto rt = dt;
       rt.lds_n = 0;
       const uint32_t rg = std::min<uint32_t>(grid_for(a.m), 1024u);

@@ -104,8 +104,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
 }
 
 // Exact redo of the queries k_nearest_n_screen32 marked (count kRedo): the plain FP64
-// traversal and collector, one thread per marked query. Returns at once when none was
-// marked; otherwise the grid strides over the counts in input order.
+// traversal and collector, one thread per marked query; the grid strides over this call's
+// counts in input order. The decision reads only this call's own counts: the tree's redo
+// counter is shared by every call on the tree (and cleared by each), so a call running
+// concurrently on another stream could clear it between this call's kernels.
 template <int K, int MAXK>
 __global__ __launch_bounds__(kBlock) void k_nearest_n_redo(DevTree<double, K> t, const double* __restrict__ queries,
                                                            uint64_t m, int k, double radius, int excl,
@@ -113,7 +115,6 @@ __global__ __launch_bounds__(kBlock) void k_nearest_n_redo(DevTree<double, K> t,
                                                            double* __restrict__ out_d,
                                                            uint32_t* __restrict__ out_counts,
                                                            unsigned long long* __restrict__ stats) {
-  if (stats[kRedoWord] == 0ull) return;
   Counters cnt;
   const uint64_t stride = uint64_t(gridDim.x) * kBlock;
   for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride) {

@@ -393,3 +393,30 @@ def test_screened_redo_pass(ka, oracle_mod, cuda):
     tu = ka.ImmutableKdTree.new_from_slice(u, 32, shadow_f32=True)
     tu.nearest_n_raw(rng.random((4000, 3)), 16, math.inf, ka.KD_SCREEN_F32 | ka.KD_MORTON)
     assert tu.query_redos() == 0
+
+
+def test_screened_redo_concurrent_streams(ka, oracle_mod, cuda):
+    """Screened calls on one tree from two streams at once, on a tree whose queries go to the
+    redo pass: the redo pass reads only its own call's counts, so neither call can lose the
+    other's marks when the tree's shared counters are cleared. Every call equals the oracle."""
+    import torch
+    rng = np.random.default_rng(11)
+    pts = rng.integers(0, 7, (40_000, 3)).astype(np.float64)
+    t = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
+    O = OracleEngine(oracle_mod)
+    ot = O.build(pts, 32)
+    qs = [rng.random((30_000, 3)) * 6.0 for _ in range(2)]
+    refs = [O.nearest_n(ot, q, 16) for q in qs]
+    qd = [torch.from_numpy(q).cuda() for q in qs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for rep in range(4):
+        _announce("screened redo streams", rep)
+        outs = []
+        for s, q in zip(streams, qd):
+            with torch.cuda.stream(s):
+                outs.append(t.nearest_n_raw(q, 16, math.inf, ka.KD_SCREEN_F32 | ka.KD_MORTON, stream=s))
+        torch.cuda.synchronize()
+        for r, ref in zip(outs, refs):
+            got = (r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy())
+            for a, b, nm in zip(got, ref, ("items", "dists", "counts")):
+                assert_same_bits(a, b, f"streams rep={rep} {nm}")
