@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--rates", default="2000,8000,16000,32000",
                     help="serving: arrival rates (per second) of the rate sweep before the main run")
     ap.add_argument("--trees", type=int, default=4096)
+    ap.add_argument("--forest-bucket", type=int, default=0,
+                    help="forest: bucket size of every tree (0: autotuned on every 16th tree)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -397,8 +399,20 @@ def bench_forest(args, torch, ka):
         pts = list(ex.map(lambda i: noisy_sphere(int(sizes[i]), seed=10_000 + i), range(T)))
         qs = list(ex.map(lambda i: noisy_sphere(int(nq[i]), seed=50_000 + i), range(T)))
         log(f"forest data: {sizes.sum()} points, {nq.sum()} queries in {time.time() - t0:.1f}s")
+        # bucket size by measurement on every 16th tree and its queries (kd_forest_autotune_bucket)
+        bucket, bucket_ms = args.forest_bucket, None
+        if bucket == 0:
+            t0 = time.time()
+            smp = list(range(0, T, 16))
+            soffs = np.zeros(len(smp) + 1, np.uint64)
+            soffs[1:] = np.cumsum([nq[i] for i in smp])
+            sq = torch.from_numpy(np.concatenate([qs[i] for i in smp])).cuda()
+            bucket, bucket_ms = ka.Forest.autotune_bucket([pts[i] for i in smp], sq, soffs, 8,
+                                                          buckets=(16, 32, 64, 128))
+            del sq
+            log(f"bucket autotuned on {len(smp)} trees in {time.time() - t0:.1f}s: {bucket} {bucket_ms}")
         t0 = time.time()
-        lays = list(ex.map(lambda p: ka.HostLayout(p, args.bucket, None, 1), pts))
+        lays = list(ex.map(lambda p: ka.HostLayout(p, bucket, None, 1), pts))
     log(f"host build {time.time() - t0:.1f}s")
     t0 = time.time()
     forest = ka.Forest(lays)
@@ -442,7 +456,7 @@ def bench_forest(args, torch, ka):
         sample = list(range(0, T, max(1, T // 64)))
         cpu_q, cpu_dt = 0, 0.0
         for i in sample:
-            ot = O.Tree.build(pts[i], args.bucket)
+            ot = O.Tree.build(pts[i], bucket)
             t0 = time.perf_counter()
             ot.nearest_n(qs[i], k, mode=O.FAITHFUL, threads=threads)
             cpu_dt += time.perf_counter() - t0
@@ -459,7 +473,8 @@ def bench_forest(args, torch, ka):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded noisy unit-sphere clouds)",
         "config": {"workload": "forest-4096-f32-knn8", "trees": T, "points": int(sizes.sum()),
-                   "queries": m, "k": k, "launch": chosen},
+                   "queries": m, "k": k, "launch": chosen, "bucket": bucket,
+                   "bucket_autotune_ms": bucket_ms},
         "within_r0.02": {"ms": round(wdt * 1e3, 2), "results": nres,
                          "results_per_s": round(nres / wdt, 1)},
         "roofline": {"bound": "hbm", "achieved": round(algo / kern_s / 1e9, 1), "peak": HBM_PEAK_GBS,

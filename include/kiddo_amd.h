@@ -182,6 +182,17 @@ int kd_autotune_within(kd_tree* tree, const void* queries, uint64_t m, double ra
  * chosen4 = {variant, group_size, lds_levels, waves}. Synchronous. */
 int kd_autotune_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k,
                           double radius, uint32_t flags, int32_t* chosen4, void* stream);
+/* Bucket size picked by measurement for one tree built on the device from n points in HBM
+ * (kd_tree_build_device arguments; no Kiddo counterpart: the bucket is ImmutableKdTree's
+ * const generic B, construction.rs:278-303, and results do not depend on it): the tree is
+ * built at each of the nbuckets candidates (1..1024), its k-NN launch shape autotuned and
+ * nearest_n(k, radius, flags) timed on the batch's tuning sample (up to 2^20 queries,
+ * Morton-coherent under KD_MORTON); *best_bucket receives the fastest, ms_out[i] (optional)
+ * each candidate's kernel time. The caller then builds the tree at *best_bucket. */
+int kd_autotune_bucket(int32_t dtype, int32_t dims, uint64_t n, const void* points, const uint32_t* items,
+                       uint32_t tree_flags, const void* queries, uint64_t m, uint32_t k, double radius,
+                       uint32_t flags, const int32_t* buckets, uint32_t nbuckets, int32_t* best_bucket,
+                       float* ms_out, void* stream);
 
 /* --------------------------------------------------------------------- queries ---- */
 /* tree.query(q).nearest_one::<SquaredEuclidean>().execute()  (builder.rs:2008-2017;
@@ -318,6 +329,17 @@ int kd_forest_autotune_nearest_n(kd_forest* forest, const void* queries, uint32_
 int kd_forest_set_timing(kd_forest* forest, int32_t enable);
 int kd_forest_last_kernel_ms(kd_forest* forest, float* ms);
 int kd_forest_query_stats(kd_forest* forest, uint64_t* out4);
+/* Bucket size picked by measurement for a batched forest (no Kiddo counterpart: the bucket
+ * is ImmutableKdTree's const generic B, construction.rs:278-303, and results do not depend on
+ * it). The ntrees trees (host point arrays, points[i] of sizes[i] x dims, typically a
+ * sample of the forest's trees) are built at each of the nbuckets candidates, their
+ * concatenated device queries planned (query_offsets: ntrees + 1 host values), the launch
+ * shape autotuned and nearest_n(k, radius) timed; *best_bucket receives the fastest and
+ * ms_out[i] (optional) each candidate's kernel time. k <= 64. */
+int kd_forest_autotune_bucket(int32_t dtype, int32_t dims, uint32_t ntrees, const void* const* points,
+                              const uint64_t* sizes, const void* queries, const uint64_t* query_offsets,
+                              uint32_t k, double radius, uint32_t flags, const int32_t* buckets,
+                              uint32_t nbuckets, int32_t* best_bucket, float* ms_out, void* stream);
 
 /* ------------------------------------------------------------------- serving ---- */
 /* A continuous-batching request loop over resident trees (no Kiddo counterpart; it serves

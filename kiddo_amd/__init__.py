@@ -106,6 +106,10 @@ SIGNATURES = {
     "kd_tree_tune_within": ([_vp, _i32, _i32], C.c_int),
     "kd_autotune_within": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp], C.c_int),
     "kd_autotune_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp], C.c_int),
+    "kd_autotune_bucket": ([_i32, _i32, _u64, _vp, _vp, _u32, _vp, _u64, _u32, _f64, _u32, _vp, _u32, _vp, _vp,
+                            _vp], C.c_int),
+    "kd_forest_autotune_bucket": ([_i32, _i32, _u32, _vp, _vp, _vp, _vp, _u32, _f64, _u32, _vp, _u32, _vp, _vp,
+                                   _vp], C.c_int),
     "kd_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_approx_nearest_one": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], C.c_int),
     "kd_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp, _vp, _vp], C.c_int),
@@ -343,6 +347,35 @@ class ImmutableKdTree:
                                           KD_TREE_SHADOW_F32 if shadow_f32 else 0, C.byref(h),
                                           _stream_ptr(stream)), "kd_tree_build_device")
         return cls(None, _handle=h)
+
+    @classmethod
+    def autotune_bucket(cls, points, queries, k: int, radius: float = math.inf, flags: int = 0,
+                        buckets=(16, 32, 64, 128), items=None, shadow_f32: bool = False, stream=None):
+        """Bucket size by measurement (kd_autotune_bucket): the tree is built on the GPU at
+        each candidate bucket, its k-NN launch autotuned and nearest_n timed on the batch's
+        tuning sample. Returns (best_bucket, {bucket: kernel ms})."""
+        torch = _torch()
+        if isinstance(points, np.ndarray):
+            points = torch.from_numpy(np.ascontiguousarray(points)).to("cuda")
+        points = points.contiguous()
+        code = {torch.float32: KD_F32, torch.float64: KD_F64, torch.uint16: KD_U16,
+                torch.uint32: KD_U32}[points.dtype]
+        n, K = points.shape
+        it = None
+        if items is not None:
+            if isinstance(items, np.ndarray):
+                items = torch.from_numpy(np.ascontiguousarray(items, dtype=np.uint32)).to("cuda")
+            it = items.contiguous()
+        q = queries if not isinstance(queries, np.ndarray) else torch.from_numpy(np.ascontiguousarray(queries)).cuda()
+        q = q.contiguous()
+        b = np.ascontiguousarray(buckets, dtype=np.int32)
+        ms = np.zeros(b.size, np.float32)
+        best = C.c_int32()
+        _check(lib().kd_autotune_bucket(code, K, n, points.data_ptr(), it.data_ptr() if it is not None else None,
+                                        KD_TREE_SHADOW_F32 if shadow_f32 else 0, q.data_ptr(), q.shape[0], k,
+                                        float(radius), flags, _np_ptr(b), b.size, C.byref(best), _np_ptr(ms),
+                                        _stream_ptr(stream)), "kd_autotune_bucket")
+        return int(best.value), {int(x): float(y) for x, y in zip(b, ms)}
 
     def export_layout(self) -> "HostLayout":
         """Kiddo's host layout (stems, extents, arena) of this device tree."""
@@ -634,6 +667,29 @@ class Forest:
     @property
     def out_dtype(self):
         return _O_NP[self.dtype_code]
+
+    @staticmethod
+    def autotune_bucket(point_sets, queries, query_offsets, k: int, radius: float = math.inf, flags: int = 0,
+                        buckets=(16, 32, 64, 128), stream=None):
+        """Bucket size of a batched forest by measurement (kd_forest_autotune_bucket): the
+        given trees (host (n_i, K) arrays, e.g. a sample of the forest's) are built at each
+        candidate bucket, planned with their concatenated device queries (query_offsets: one
+        more than trees), the launch autotuned and nearest_n timed. Returns
+        (best_bucket, {bucket: kernel ms})."""
+        pts = [np.ascontiguousarray(p) for p in point_sets]
+        code = _NP_CODE[pts[0].dtype]
+        K = pts[0].shape[1]
+        ptrs = (C.c_void_p * len(pts))(*[p.ctypes.data for p in pts])
+        sizes = np.ascontiguousarray([p.shape[0] for p in pts], dtype=np.uint64)
+        offs = np.ascontiguousarray(query_offsets, dtype=np.uint64)
+        b = np.ascontiguousarray(buckets, dtype=np.int32)
+        ms = np.zeros(b.size, np.float32)
+        best = C.c_int32()
+        _check(lib().kd_forest_autotune_bucket(code, K, len(pts), ptrs, _np_ptr(sizes), queries.data_ptr(),
+                                               _np_ptr(offs), k, float(radius), flags, _np_ptr(b), b.size,
+                                               C.byref(best), _np_ptr(ms), _stream_ptr(stream)),
+               "kd_forest_autotune_bucket")
+        return int(best.value), {int(x): float(y) for x, y in zip(b, ms)}
 
     def plan(self, queries, query_offsets, chunk=256, morton=True, stream=None):
         """queries: concatenated (m, K) device tensor; query_offsets: ntrees+1 host ints."""

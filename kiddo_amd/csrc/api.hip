@@ -1114,6 +1114,62 @@ int kd_autotune_nearest_n(kd_tree* t, const void* queries, uint64_t m, uint32_t 
   return KD_OK;
 }
 
+// Bucket size by measurement: the tree is built on the device at each candidate bucket (the
+// canonical answer does not depend on it), its launch shape autotuned, and nearest_n timed on
+// the tuning sample of the batch (the same sample kd_autotune_nearest_n uses, up to 2^20
+// queries); the fastest bucket wins.
+int kd_autotune_bucket(int32_t dtype, int32_t dims, uint64_t n, const void* points, const uint32_t* items,
+                       uint32_t tree_flags, const void* queries, uint64_t m, uint32_t k, double radius,
+                       uint32_t flags, const int32_t* buckets, uint32_t nbuckets, int32_t* best_bucket,
+                       float* ms_out, void* stream) {
+  if (!points || !queries || !buckets || !best_bucket) return kd::set_error(KD_ERR_INVALID, "null argument");
+  if (nbuckets == 0) return kd::set_error(KD_ERR_INVALID, "no candidate bucket");
+  if (k == 0) return kd::set_error(KD_ERR_INVALID, "k must be >= 1");
+  if (m == 0) return kd::set_error(KD_ERR_INVALID, "no queries");
+  for (uint32_t i = 0; i < nbuckets; ++i)
+    if (buckets[i] < 1 || buckets[i] > 1024)
+      return kd::set_error(KD_ERR_INVALID, "bucket %d is not in 1..1024", buckets[i]);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t ms = std::min<uint64_t>(m, uint64_t(1) << 20);
+  const size_t osz = ms * k * (sizeof(uint32_t) + sizeof(double)) + ms * sizeof(uint32_t) + 1024;
+  void* mem = nullptr;
+  if (hipMalloc(&mem, osz) != hipSuccess) return kd::set_error(KD_ERR_NO_MEMORY, "hipMalloc(%zu) failed", osz);
+  uint32_t* oitems = static_cast<uint32_t*>(mem);
+  double* odists = reinterpret_cast<double*>(static_cast<uint8_t*>(mem) + ((ms * k * 4 + 255) / 256) * 256);
+  uint32_t* ocounts = reinterpret_cast<uint32_t*>(odists + ms * k);
+  float best_ms = 1e30f;
+  int rc = KD_OK;
+  *best_bucket = buckets[0];
+  for (uint32_t i = 0; i < nbuckets && rc == KD_OK; ++i) {
+    kd_tree* t = nullptr;
+    rc = kd_tree_build_device(dtype, dims, buckets[i], n, points, items, tree_flags, &t, stream);
+    if (rc) break;
+    void* sample_mem = nullptr;
+    const void* sample = queries;
+    rc = kd::tuning_sample(t, queries, m, ms, flags, s, &sample_mem, &sample);
+    if (rc == KD_OK) rc = kd_autotune_nearest_n(t, sample, ms, k, radius, flags, nullptr, stream);
+    if (rc == KD_OK) rc = kd_tree_set_timing(t, 1);
+    float tm = 1e30f;
+    for (int rep = 0; rep < 2 && rc == KD_OK; ++rep) {
+      float e = 0.0f;
+      rc = kd_nearest_n(t, sample, ms, k, radius, flags, oitems, odists, ocounts, stream);
+      if (rc == KD_OK) rc = kd_last_kernel_ms(t, &e);
+      tm = std::min(tm, e);
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(sample_mem);
+    kd_tree_destroy(t);
+    if (rc) break;
+    if (ms_out) ms_out[i] = tm;
+    if (tm < best_ms) {
+      best_ms = tm;
+      *best_bucket = buckets[i];
+    }
+  }
+  (void)hipFree(mem);
+  return rc;
+}
+
 int kd_autotune_within(kd_tree* t, const void* queries, uint64_t m, double radius, uint32_t flags,
                        int32_t* chosen2, void* stream) {
   if (int rc = kd::check_tree(t)) return rc;

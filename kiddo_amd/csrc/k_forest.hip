@@ -866,4 +866,73 @@ int kd_forest_autotune_nearest_n(kd_forest* f, const void* queries, uint32_t k, 
   return rc;
 }
 
+// Bucket size of a batched forest by measurement: the given trees (a representative sample of
+// the forest) are built on the host at each candidate bucket (threads over trees), uploaded as
+// a forest, their queries planned (chunks of 256, Morton order inside each tree when flags
+// hold KD_MORTON), the launch shape autotuned and nearest_n timed; the fastest bucket wins.
+int kd_forest_autotune_bucket(int32_t dtype, int32_t dims, uint32_t ntrees, const void* const* points,
+                              const uint64_t* sizes, const void* queries, const uint64_t* query_offsets,
+                              uint32_t k, double radius, uint32_t flags, const int32_t* buckets,
+                              uint32_t nbuckets, int32_t* best_bucket, float* ms_out, void* stream) {
+  if (!points || !sizes || !queries || !query_offsets || !buckets || !best_bucket)
+    return kd::set_error(KD_ERR_INVALID, "null argument");
+  if (ntrees == 0 || nbuckets == 0) return kd::set_error(KD_ERR_INVALID, "no trees or no candidate bucket");
+  if (k == 0 || k > uint32_t(kd::kMaxK)) return kd::set_error(KD_ERR_INVALID, "k must be in 1..%d", kd::kMaxK);
+  for (uint32_t i = 0; i < nbuckets; ++i)
+    if (buckets[i] < 1 || buckets[i] > 1024)
+      return kd::set_error(KD_ERR_INVALID, "bucket %d is not in 1..1024", buckets[i]);
+  const uint64_t m = query_offsets[ntrees];
+  if (m == 0) return kd::set_error(KD_ERR_INVALID, "no queries");
+  const size_t osz = m * k * (sizeof(uint32_t) + sizeof(double)) + m * sizeof(uint32_t) + 1024;
+  void* mem = nullptr;
+  if (hipMalloc(&mem, osz) != hipSuccess) return kd::set_error(KD_ERR_NO_MEMORY, "hipMalloc(%zu) failed", osz);
+  uint32_t* oitems = static_cast<uint32_t*>(mem);
+  void* odists = static_cast<uint8_t*>(mem) + ((m * k * 4 + 255) / 256) * 256;
+  uint32_t* ocounts = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(odists) + m * k * sizeof(double));
+  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  float best_ms = 1e30f;
+  int rc = KD_OK;
+  *best_bucket = buckets[0];
+  for (uint32_t b = 0; b < nbuckets && rc == KD_OK; ++b) {
+    std::vector<kd_layout*> lays(ntrees, nullptr);
+    std::vector<int> rcs(ntrees, KD_OK);
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nth; ++w)
+      pool.emplace_back([&, w]() {
+        for (uint32_t i = w; i < ntrees; i += nth)
+          rcs[i] = kd_build_layout(dtype, dims, buckets[b], sizes[i], points[i], nullptr, 1, &lays[i]);
+      });
+    for (auto& th : pool) th.join();
+    for (uint32_t i = 0; i < ntrees && rc == KD_OK; ++i) rc = rcs[i];
+    std::vector<kd_tree_desc> descs(ntrees);
+    for (uint32_t i = 0; i < ntrees && rc == KD_OK; ++i) rc = kd_layout_desc(lays[i], &descs[i]);
+    kd_forest* f = nullptr;
+    if (rc == KD_OK) rc = kd_forest_create(descs.data(), ntrees, 0, &f);
+    for (kd_layout* l : lays)
+      if (l) kd_layout_free(l);
+    if (rc) break;
+    rc = kd_forest_plan(f, queries, query_offsets, 256, flags & KD_MORTON, stream);
+    if (rc == KD_OK)
+      rc = kd_forest_autotune_nearest_n(f, queries, k, radius, flags, oitems, odists, ocounts, nullptr, stream);
+    if (rc == KD_OK) rc = kd_forest_set_timing(f, 1);
+    float tm = 1e30f;
+    for (int rep = 0; rep < 2 && rc == KD_OK; ++rep) {
+      float e = 0.0f;
+      rc = kd_forest_nearest_n(f, queries, k, radius, flags & ~uint32_t(KD_GRAPH), oitems, odists, ocounts, stream);
+      if (rc == KD_OK) rc = kd_forest_last_kernel_ms(f, &e);
+      tm = std::min(tm, e);
+    }
+    (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    kd_forest_destroy(f);
+    if (rc) break;
+    if (ms_out) ms_out[b] = tm;
+    if (tm < best_ms) {
+      best_ms = tm;
+      *best_bucket = buckets[b];
+    }
+  }
+  (void)hipFree(mem);
+  return rc;
+}
+
 }  // extern "C"

@@ -50,6 +50,23 @@ def test_error_reporting_without_gpu(ka):
     assert rc == -4  # more than 1024 equal points end in one leaf (construction.rs:526)
     # null-tree query calls fail cleanly
     assert lib.kd_nearest_n(None, None, 1, 4, float("inf"), 0, None, None, None, None) == -1
+    # the bucket autotuners check their arguments before touching the device
+    best = C.c_int32()
+    b = np.array([0, 32], np.int32)
+    dummy = C.c_void_p(8)
+    rc = lib.kd_autotune_bucket(1, 3, 4, dummy, None, 0, dummy, 4, 4, float("inf"), 0,
+                                b.ctypes.data_as(C.c_void_p), 2, C.byref(best), None, None)
+    assert rc == -1 and b"bucket 0" in lib.kd_last_error()
+    rc = lib.kd_autotune_bucket(1, 3, 4, dummy, None, 0, dummy, 4, 4, float("inf"), 0,
+                                b.ctypes.data_as(C.c_void_p), 0, C.byref(best), None, None)
+    assert rc == -1
+    offs = np.array([0, 4], np.uint64)
+    sizes = np.array([4], np.uint64)
+    ptrs = (C.c_void_p * 1)(pts.ctypes.data)
+    rc = lib.kd_forest_autotune_bucket(1, 3, 1, ptrs, sizes.ctypes.data_as(C.c_void_p), dummy,
+                                       offs.ctypes.data_as(C.c_void_p), 65, float("inf"), 0,
+                                       b.ctypes.data_as(C.c_void_p), 2, C.byref(best), None, None)
+    assert rc == -1 and b"k must be" in lib.kd_last_error()
 
 
 def _layout_equal(ka, O, pts, B, items=None, threads=4):
