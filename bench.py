@@ -695,6 +695,12 @@ def bench_mixture(args, torch, ka):
     log(f"mixture within: {total} results in {chunks} chunks, count {t_count * 1e3:.0f} ms, "
         f"total {t_within * 1e3:.0f} ms ({total / t_within / 1e9:.2f} G results/s)")
     algo = st["points"] * 16 + st["stems"] * 4 + m * 12 + m * 16 * 8
+    # measured HBM traffic of these kernels (rocprofv3 FETCH_SIZE / WRITE_SIZE, committed)
+    tr = {}
+    tpath = os.path.join(ROOT, "profiles", "mixture_10M_f32_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
     print(json.dumps({
         "metric": "queries/s, 10M-pt f32 Gaussian-mixture kNN k=16; within(r=0.005) results/s",
         "value": round(m / dt, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -715,17 +721,20 @@ def bench_mixture(args, torch, ka):
                                                      "achieved": round(fill_bytes / (fill_kernel_ms / 1e3) / 1e9, 1),
                                                      "frac": round(fill_bytes / (fill_kernel_ms / 1e3) / 1e9
                                                                    / HBM_PEAK_GBS, 4),
-                                                     "algo_bytes": int(fill_bytes), "traffic": None,
+                                                     "algo_bytes": int(fill_bytes),
+                                                     "traffic": tr.get("k_within_fill_group", {}).get("bytes_per_pass"),
                                                      "kernel": "k_within_fill_group"},
                                    "roofline_count": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                                                       "achieved": round(count_bytes / (count_kernel_ms / 1e3) / 1e9, 1),
                                                       "frac": round(count_bytes / (count_kernel_ms / 1e3) / 1e9
                                                                     / HBM_PEAK_GBS, 4),
-                                                      "algo_bytes": int(count_bytes), "traffic": None,
+                                                      "algo_bytes": int(count_bytes),
+                                                      "traffic": tr.get("k_within_count_group", {}).get("bytes_per_launch"),
                                                       "kernel": "k_within_count_group"}},
         "roofline": {"bound": "hbm", "achieved": round(algo / (kms / 1e3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(algo / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(algo / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": tr.get("k_nearest_n<float,3,16>", {}).get("bytes_per_launch"),
                      "kernel": "k_nearest_n<float,3,16>", "kernel_ms": round(kms, 3)},
     }), flush=True)
 

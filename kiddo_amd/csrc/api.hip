@@ -1208,7 +1208,9 @@ int kd_autotune_within(kd_tree* t, const void* queries, uint64_t m, double radiu
       best_count = g;
     }
   }
-  // the fill pass, on as many of the sample's queries as 2^28 results allow
+  // the fill pass, on as many of the sample's queries as 2^29 results allow (with 2^28, a
+  // dense workload's fill sample was ~150k queries, too small a launch to rank the shapes:
+  // the pick flipped between 8 and 16 lanes, 178 against 277 ms at full size)
   int best_fill = 1;
   uint64_t mf = ms, total = 0;
   std::vector<uint64_t> h(ms + 1);
@@ -1218,7 +1220,7 @@ int kd_autotune_within(kd_tree* t, const void* queries, uint64_t m, double radiu
       rc = kd::set_error(KD_ERR_DEVICE, "offsets copy failed");
   }
   if (rc == KD_OK) {
-    constexpr uint64_t kBudget = uint64_t(1) << 28;
+    constexpr uint64_t kBudget = uint64_t(1) << 29;
     mf = uint64_t(std::upper_bound(h.begin(), h.end(), kBudget) - h.begin()) - 1;
     mf = std::max<uint64_t>(std::min<uint64_t>(mf, ms), 1);
     total = h[mf];

@@ -1656,6 +1656,79 @@ template <class O, int G> struct GroupWithinVis {
   }
 };
 
+// Group within fill with staged writes: the G lanes of a query write the hits of each step
+// into a ring of kRing results in LDS (position = result index mod kRing), and the group
+// stores the ring to global memory in whole segments that end on a multiple of 32 results
+// (128 bytes of items), so the L2 receives complete lines from one wave at a time. Writing
+// each step's few hits straight to global memory left lines partly written when the L2
+// evicted them: 2.4 bytes of HBM writes per result byte on a dense workload. Only the
+// query's first and last lines are partial. Every lane of the group must call leaf() and
+// finish(); lanes of one group are in one wave, so a wave barrier orders the ring's stores
+// before its loads.
+template <class O, int G> struct GroupWithinStageVis {
+  static constexpr uint32_t kRing = 64;
+  static constexpr uint32_t kAlign = 32;
+  O radius;
+  bool excl;
+  int gl;
+  uint64_t count;    // results so far (group-uniform)
+  uint64_t flushed;  // results already stored to global memory
+  uint64_t base;     // the query's first result index in the output arrays (alignment)
+  uint32_t* out_items;  // the query's segment
+  O* out_d;
+  uint64_t cap;
+  uint32_t* ring_it;  // this group's ring (LDS)
+  O* ring_d;
+  __device__ O bound() const { return radius; }
+  __device__ bool prune(O rd) const { return rd > radius || (excl && !(rd < radius)); }
+  __device__ bool prune_at(O rd, O b) const { return rd > b || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return radius == oinf<O>(); }
+  // stores results [flushed, end) (end = the last 32-result boundary up to count, or count
+  // when final)
+  __device__ __forceinline__ void flush(bool final) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint64_t end = count;
+    if (!final) {
+      const uint64_t a = ((base + count) / kAlign) * kAlign;
+      end = a > base + flushed ? a - base : flushed;
+    }
+    for (uint64_t j = flushed + uint64_t(gl); j < end; j += G) {
+      if (j < cap) {
+        out_items[j] = ring_it[j % kRing];
+        out_d[j] = ring_d[j % kRing];
+      }
+    }
+    flushed = end;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf, uint32_t nl = 1u) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + nl];
+    const int first = int(threadIdx.x & 63u) - gl;  // first lane of this group in the wave
+    for (uint32_t p0 = s; p0 < e; p0 += G) {
+      const uint32_t p = p0 + uint32_t(gl);
+      O d = O(0);
+      bool hit = false;
+      if (p < e) {
+        d = point_dist(t, p, qw);
+        hit = excl ? d < radius : d <= radius;
+      }
+      const uint64_t mine = (uint64_t(__ballot(hit)) >> first) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull));
+      if (hit) {
+        const uint64_t o = count + uint64_t(__popcll(mine & ((1ull << gl) - 1ull)));
+        ring_it[o % kRing] = t.items[p];
+        ring_d[o % kRing] = d;
+      }
+      count += uint64_t(__popcll(mine));
+      if (count - flushed > kRing - G) flush(false);  // room for the next step's G hits
+    }
+  }
+};
+
 // Group k-NN collector (k <= G). SCREEN: f64 trees measured first in FP32 from the shadow
 // against the certified bound of screen_threshold, exact FP64 only for survivors.
 template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {

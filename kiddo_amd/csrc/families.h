@@ -249,6 +249,9 @@ __global__ __launch_bounds__(kBlock) void k_within_fill_group(DevTree<A, K> t, c
                                                               uint32_t* __restrict__ out_items,
                                                               typename Ax<A>::O* __restrict__ out_d) {
   using O = typename Ax<A>::O;
+  using V = GroupWithinStageVis<O, G>;
+  __shared__ uint32_t ring_it[(kBlock / G) * V::kRing];
+  __shared__ O ring_d[(kBlock / G) * V::kRing];
   pin_stems(t);
   const uint64_t gid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / G;
   const int gl = int(threadIdx.x) & (G - 1);
@@ -258,9 +261,11 @@ __global__ __launch_bounds__(kBlock) void k_within_fill_group(DevTree<A, K> t, c
   Small<O, K> qw;
   load_query<A, K>(queries, qi, q, qw);
   const uint64_t o = offsets[qi], cap = offsets[qi + 1] - o;
-  GroupWithinVis<O, G> v{radius, excl != 0, gl, 0, out_items + o, out_d + o, cap};
+  const uint32_t ring = (threadIdx.x / G) * V::kRing;
+  V v{radius, excl != 0, gl, 0, 0, o, out_items + o, out_d + o, cap, ring_it + ring, ring_d + ring};
   Counters c;
   traverse(t, q, qw, v, c);
+  v.flush(true);
   if (sorted && gl == 0) sort_segment(out_items + o, out_d + o, v.count < cap ? v.count : cap);
 }
 
