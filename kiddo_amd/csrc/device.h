@@ -1192,7 +1192,7 @@ __device__ __forceinline__ float round_up_f32(double x) {
 //     entries are screened again at the end.
 //   * In later visits, an entry whose FP32 distance passes thr (every entry whose FP64
 //     distance can be at most the bound passes) is appended to a list of candidates in LDS
-//     (kCand entry indices per lane). When the list is full it is compacted against the
+//     (kCand = k + 16 entry indices per lane). When the list is full it is compacted against the
 //     current threshold: an entry of the answer passes every threshold, so none is dropped.
 //     A list that is still full after compaction cannot prove it holds the whole answer:
 //     the query is marked for the exact redo pass (k_nearest_n_redo) and its traversal ends
@@ -1204,7 +1204,9 @@ __device__ __forceinline__ float round_up_f32(double x) {
 // FP64 measurement and an insertion into an FP64 (distance, item) list.
 template <int MAXK> struct Screen32Vis {
   static_assert(MAXK <= 16, "the FP32 collector is for k <= 16");
-  static constexpr uint32_t kCand = MAXK == 1 ? 8u : uint32_t(MAXK) + 8u;
+  // candidate list entries per lane: k + 16 (fewer compactions than k + 8: 15.4 against
+  // 16.0 ms on 16M queries at k = 16, with the LDS stems cut to 8 KiB to keep 4 blocks per CU)
+  static constexpr uint32_t kCand = MAXK == 1 ? 8u : uint32_t(MAXK) + 16u;
   static constexpr int kWords = MAXK >= 16 ? 2 : 1;
   static constexpr uint32_t kMaskN = 64u * kWords;
   static constexpr float kC = 1.0f + 0x1p-16f;
@@ -1282,15 +1284,29 @@ template <int MAXK> struct Screen32Vis {
     while (next < L && screen_dist(t, s + next) > thr) ++next;
     return next;
   }
-  // keeps the candidates that pass the current threshold, in order
+  // keeps the candidates that pass the current threshold, in order; 8 at a time, with their
+  // shadow loads in flight together (one at a time, a compaction was a chain of 24 dependent
+  // loads, which the wave of a lane whose list filled waited for)
   template <int K> __device__ __forceinline__ void compact(const DevTree<double, K>& t) {
     const uint32_t lane = threadIdx.x;
+    constexpr uint32_t kChunk = 8;
     uint32_t w = 0;
-    for (uint32_t j = 0; j < nc; ++j) {
-      const uint32_t p = cand[j * kBlock + lane];
-      const bool keep = !(screen_dist(t, p) > thr);
-      cand[w * kBlock + lane] = p;
-      w += keep ? 1u : 0u;
+    for (uint32_t j0 = 0; j0 < nc; j0 += kChunk) {
+      uint32_t p[kChunk];
+      float4 sh[kChunk];
+#pragma unroll
+      for (uint32_t u = 0; u < kChunk; ++u) p[u] = cand[(j0 + u < nc ? j0 + u : j0) * kBlock + lane];
+#pragma unroll
+      for (uint32_t u = 0; u < kChunk; ++u) sh[u] = t.shadow[p[u]];
+#pragma unroll
+      for (uint32_t u = 0; u < kChunk; ++u) {
+        const bool keep = (j0 + u < nc) & !(shadow_sq<K>(sh[u], qf) > thr);
+        // only kept entries are stored: w <= j0 + u < nc <= kCand, a slot already read. (An
+        // unconditional store of the chunk's positions past nc reached slot kCand, past the
+        // array, whenever the list was full, kept everything and kCand was not a multiple of 8.)
+        if (keep) cand[w * kBlock + lane] = p[u];
+        w += keep ? 1u : 0u;
+      }
     }
     nc = w;
   }

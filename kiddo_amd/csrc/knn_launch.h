@@ -200,9 +200,14 @@ int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, v
   KernelTimer kt(t, a.s);
   if constexpr (std::is_same<A, double>::value) {
     if ((a.flags & KD_SCREEN_F32) && MAXK <= 16) {
+      // LDS: the candidate list plus the stems within 40 KiB per block, so that 4 blocks (16
+      // waves) fit a CU's 160 KiB: the stems take the largest power of two that fits
+      constexpr size_t kList = size_t(dev::Screen32Vis<(MAXK <= 16 ? MAXK : 16)>::kCand) * 4 * dev::kBlock;
+      auto st = dt;
+      while (st.lds_n && kList + size_t(st.lds_n) * sizeof(double) > 40960) st.lds_n >>= 1;
       hipLaunchKernelGGL((dev::k_nearest_n_screen32<K, (MAXK <= 16 ? MAXK : 16), WAVES>), dim3(grid_for(a.m)),
-                         dim3(dev::kBlock), lds, a.s, dt, q, a.order, a.m, k, radius, excl, t->d_max_abs, items,
-                         static_cast<double*>(dists), counts, t->counters);
+                         dim3(dev::kBlock), size_t(st.lds_n) * sizeof(double), a.s, st, q, a.order, a.m, k, radius,
+                         excl, t->d_max_abs, items, static_cast<double*>(dists), counts, t->counters);
       KD_HIP(hipGetLastError());
       // the queries whose candidate list overflowed, answered exactly; global stems only
       auto rt = dt;
