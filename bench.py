@@ -440,13 +440,16 @@ def bench_forest(args, torch, ka):
     dt = time.perf_counter() - t0
     st = forest.query_stats()
     log(f"forest knn: {1e3 * dt / args.steps:.1f} ms/step, {m * args.steps / dt / 1e6:.1f} Mq/s, {st}")
-    # within(r = 0.02): count + scan + fill, timed together
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    w = forest.within(q, 0.02 * 0.02, ka.KD_UNSORTED)
-    torch.cuda.synchronize()
-    wdt = time.perf_counter() - t1
-    nres = int(w.offsets[m].item())
+    # within(r = 0.02): count + scan + fill, timed together, best of 3
+    wdt = float("inf")
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        w = forest.within(q, 0.02 * 0.02, ka.KD_UNSORTED)
+        torch.cuda.synchronize()
+        wdt = min(wdt, time.perf_counter() - t1)
+        nres = int(w.offsets[m].item())
+        del w
     log(f"forest within: {wdt * 1e3:.1f} ms, {nres} results")
     cpu = None
     if not args.no_cpu:
