@@ -469,6 +469,11 @@ def bench_forest(args, torch, ka):
                           f"{cpu_q} queries, nearest_n(k=8), query time only")
     kern_s = float(np.mean(kms)) / 1e3
     algo = st["points"] * (3 * 4 + 4) + st["stems"] * 4 + m * 3 * 4 + m * k * 8
+    tpath = os.path.join(ROOT, "profiles", "forest_4096_f32_traffic.json")
+    ftraffic = None
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            ftraffic = json.load(fh).get("k_forest_knn", {}).get("bytes_per_launch")
     print(json.dumps({
         "metric": "queries/s, 4096-tree batched kNN k=8 (f32 3-D, persistent hipGraph launch)",
         "value": round(m * args.steps / dt, 1), "unit": "queries/s", "n_gpus": 1,
@@ -482,7 +487,7 @@ def bench_forest(args, torch, ka):
                          "results_per_s": round(nres / wdt, 1)},
         "roofline": {"bound": "hbm", "achieved": round(algo / kern_s / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(algo / kern_s / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_forest_knn", "kernel_ms": round(kern_s * 1e3, 3)},
+                     "traffic": ftraffic, "kernel": "k_forest_knn", "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }), flush=True)
 
