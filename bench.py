@@ -599,6 +599,9 @@ def bench_serving(args, torch, ka):
     tail = [{"arrival_ms": round(float(arrivals[i]) * 1e3, 2), "latency_ms": round(float(lat[i]), 2),
              "queries": int(sizes[i]), "family": names[reqs[i][1]]} for i in sorted(slow)]
     log(f"slowest requests: {tail}")
+    roof = _serving_replay_roofline(reqs, [sphere, palette], args, torch, ka, fams, names)
+    roof["gpu_busy_fraction"] = round(roof["kernel_ms"] / 1e3 / (end - start), 4)
+    log(f"serving replay roofline: {roof}")
     print(json.dumps({
         "metric": "queries/s and p50/p99 request latency, continuous batching over resident trees",
         "value": round(value, 1), "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -612,7 +615,68 @@ def bench_serving(args, torch, ka):
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "per_family": by_fam, "per_size": by_size, "slowest": tail, "server": st,
         "offered_qps": round(total_q / arrivals[-1], 1),
+        "roofline": roof,
     }), flush=True)
+
+
+def _serving_replay_roofline(reqs, trees, args, torch, ka, fams, names):
+    """Roofline of the serving mix's query kernels. The requests are replayed directly on the
+    trees after the server has closed, cut into parts of `max_part` queries and coalesced per
+    kind (tree, family, k, radius, flags) in arrival order into launches of at most 65,536
+    queries, the server's launches when every part of a kind is pending at once. Each launch
+    is timed with the tree's HIP events around its query kernel; its algorithmic bytes are the
+    entries it scanned (coordinates + u32 item), the stems it read, its queries and its
+    results (u32 item + distance, and a u32 count for the k-row families), from the tree's
+    visit counters. `kernel_ms` is the sum over launches; `achieved` = bytes / kernel_ms."""
+    groups = {}
+    for tree, fam, q, k, radius, flags in reqs:
+        for a in range(0, q.shape[0], args.max_part):
+            groups.setdefault((tree, fam, k, radius, flags), []).append(q[a:a + args.max_part])
+    for t in trees:
+        t.set_timing(True)
+    tot_ms, tot_bytes, launches, per = 0.0, 0, 0, {}
+    for (tree, fam, k, radius, flags), parts in groups.items():
+        t = trees[tree]
+        launch_q, cur = [], []
+        for p in parts:
+            if cur and sum(x.shape[0] for x in cur) + p.shape[0] > 1 << 16:
+                launch_q.append(np.concatenate(cur))
+                cur = []
+            cur.append(p)
+        if cur:
+            launch_q.append(np.concatenate(cur))
+        for qh in launch_q:
+            K, esz, mm = qh.shape[1], qh.dtype.itemsize, qh.shape[0]
+            q = torch.from_numpy(np.ascontiguousarray(qh)).cuda()
+            if fam == ka.FAMILY_NEAREST_ONE:
+                out = t.nearest_one_raw(q, flags)
+                res_bytes = mm * (4 + out.distance.element_size())
+            elif fam == ka.FAMILY_NEAREST_N:
+                out = t.nearest_n_raw(q, k, math.inf, flags)
+                res_bytes = mm * k * (4 + out.distances.element_size()) + mm * 4
+            else:
+                out = t.best_n_within_raw(q, radius, k, flags)
+                res_bytes = mm * k * (4 + out.distances.element_size()) + mm * 4
+            ms = t.last_kernel_ms()
+            st = t.query_stats()
+            b = st["points"] * (K * esz + 4) + st["stems"] * esz + mm * K * esz + res_bytes
+            key = ("sphere" if tree == 0 else "palette") + "/" + names[fam]
+            g = per.setdefault(key, {"launches": 0, "queries": 0, "kernel_ms": 0.0, "algo_bytes": 0})
+            g["launches"] += 1
+            g["queries"] += mm
+            g["kernel_ms"] += ms
+            g["algo_bytes"] += int(b)
+            tot_ms += ms
+            tot_bytes += b
+            launches += 1
+    for g in per.values():
+        g["achieved_gbs"] = round(g["algo_bytes"] / (g["kernel_ms"] / 1e3) / 1e9, 1)
+        g["kernel_ms"] = round(g["kernel_ms"], 3)
+    achieved = tot_bytes / (tot_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel_ms": round(tot_ms, 3),
+            "algo_bytes": int(tot_bytes), "launches": launches,
+            "kernel": "replay of the request mix, all query kernels", "per_kind": per}
 
 
 def _time_knn(tree, q, k, flags, steps, warmup, torch, out=None):
