@@ -153,8 +153,10 @@ int kd_tree_destroy(kd_tree* tree);
 int kd_tree_get_info(const kd_tree* tree, kd_tree_info* out);
 
 /* Launch tuning (no Kiddo counterpart; results never depend on it).
- * knn_variant: 0 automatic, 1 one thread per query, 2 one group of group_size lanes per
- * query (group_size 0 = smallest of 4/8/16/32/64 that holds k). lds_levels: top stem
+ * knn_variant: 0 automatic (for k <= 16, group_size lanes per query, 0 = 8 or 16, on
+ * batches below 2^16 queries, or 2^19 on trees of at least 2^22 entries; else one thread
+ * per query), 1 one thread per query, 2 one group of group_size lanes per query
+ * (group_size 0 = smallest of 4/8/16/32/64 that holds k). lds_levels: top stem
  * levels copied into LDS per block (-1 automatic, 0 none, at most 14). */
 int kd_tree_tune(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_t lds_levels);
 /* As kd_tree_tune, plus the minimum waves per SIMD the thread-per-query k-NN kernel is
@@ -168,6 +170,12 @@ int kd_tree_tune2(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_
  * one thread per query; the fill pass: 16 lanes per query when the batch averages 32 or
  * more results per query, else one thread). */
 int kd_tree_tune_within(kd_tree* tree, int32_t count_group, int32_t fill_group);
+/* Launch shape of best_n_within (no Kiddo counterpart; results never depend on it): 1 one
+ * thread per query, 4/8/16/32 lanes per query sharing each leaf visit (SquaredEuclidean;
+ * a shape with fewer lanes than n, and other metrics, use one thread per query), 0
+ * automatic (for n <= 16: 16 lanes below 2^17 queries, 8 lanes for n <= 8 and 16 for
+ * n <= 16 below 2^22 queries; one thread per query above). */
+int kd_tree_tune_best_n_within(kd_tree* tree, int32_t group);
 /* Autotuner of the within passes: times each shape of the count pass on up to 2^20 of the
  * queries (with KD_MORTON: 16 runs of their Morton order), then each shape of the fill pass
  * on as many of them as 2^28 results allow, and keeps the fastest of each in the tree

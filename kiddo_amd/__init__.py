@@ -104,6 +104,7 @@ SIGNATURES = {
     "kd_tree_tune": ([_vp, _i32, _i32, _i32], C.c_int),
     "kd_tree_tune2": ([_vp, _i32, _i32, _i32, _i32], C.c_int),
     "kd_tree_tune_within": ([_vp, _i32, _i32], C.c_int),
+    "kd_tree_tune_best_n_within": ([_vp, _i32], C.c_int),
     "kd_autotune_within": ([_vp, _vp, _u64, _f64, _u32, _vp, _vp], C.c_int),
     "kd_autotune_nearest_n": ([_vp, _vp, _u64, _u32, _f64, _u32, _vp, _vp], C.c_int),
     "kd_autotune_bucket": ([_i32, _i32, _u64, _vp, _vp, _u32, _vp, _u64, _u32, _f64, _u32, _vp, _u32, _vp, _vp,
@@ -461,6 +462,11 @@ class ImmutableKdTree:
         """Launch shapes of the within passes: 0 automatic, 1 thread per query, 4..32 lanes
         per query."""
         _check(lib().kd_tree_tune_within(self._h, count_group, fill_group), "kd_tree_tune_within")
+
+    def tune_best_n_within(self, group: int = 0):
+        """Launch shape of best_n_within: 0 automatic, 1 thread per query, 4..32 lanes per
+        query (at least n; SquaredEuclidean)."""
+        _check(lib().kd_tree_tune_best_n_within(self._h, group), "kd_tree_tune_best_n_within")
 
     def autotune_within(self, q, radius, flags=0, stream=None):
         """Times the within passes' launch shapes on a sample of q and keeps the fastest."""

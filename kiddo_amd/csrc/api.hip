@@ -729,6 +729,14 @@ int kd_tree_tune_within(kd_tree* t, int32_t count_group, int32_t fill_group) {
   return KD_OK;
 }
 
+int kd_tree_tune_best_n_within(kd_tree* t, int32_t group) {
+  if (int rc = kd::check_tree(t)) return rc;
+  if (!(group == 0 || group == 1 || group == 4 || group == 8 || group == 16 || group == 32))
+    return kd::set_error(KD_ERR_INVALID, "best_n_within launch shape must be 0, 1, 4, 8, 16 or 32");
+  t->best_group = group;
+  return KD_OK;
+}
+
 namespace kd {
 namespace {
 // Metric field of the flags (KD_METRIC_*): kind 0..3; Minkowski exponents 1 and 2 are

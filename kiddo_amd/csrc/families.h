@@ -116,6 +116,41 @@ __global__ __launch_bounds__(kBlock) void k_best_n_within(DevTree<A, K> t, const
   flush_counters(cnt, stats);
 }
 
+// best_n_within, G lanes per query (n <= G; SquaredEuclidean): more lanes for batches too
+// small to fill the device one thread per query, and each leaf's entries are loaded G at a
+// time. Lane j < n of the group writes slot j of the row.
+template <class A, int K, int G>
+__global__ __launch_bounds__(kBlock) void k_best_n_within_group(DevTree<A, K> t, const A* __restrict__ queries,
+                                                                const uint32_t* __restrict__ order, uint64_t m,
+                                                                int n, typename Ax<A>::O radius, int excl,
+                                                                uint32_t* __restrict__ out_items,
+                                                                typename Ax<A>::O* __restrict__ out_d,
+                                                                uint32_t* __restrict__ out_counts,
+                                                                unsigned long long* __restrict__ stats) {
+  using O = typename Ax<A>::O;
+  pin_stems(t);
+  const uint64_t gid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / G;
+  const int gl = int(threadIdx.x) & (G - 1);
+  const bool active = gid < m;  // uniform within a group
+  Counters c;
+  if (active) {
+    const uint64_t qi = order ? order[gid] : gid;
+    Small<A, K> q;
+    Small<O, K> qw;
+    load_query<A, K>(queries, qi, q, qw);
+    GroupBestNVis<O, G> v;
+    v.init(n, radius, excl != 0, gl);
+    traverse(t, q, qw, v, c);
+    if (gl < n) {
+      out_items[qi * uint64_t(n) + gl] = gl < v.cnt ? v.si : 0xFFFFFFFFu;
+      out_d[qi * uint64_t(n) + gl] = gl < v.cnt ? v.sd : oinf<O>();
+    }
+    if (gl == 0) out_counts[qi] = uint32_t(v.cnt);
+    if (gl != 0) c = Counters{};  // counted once per group
+  }
+  flush_counters(c, stats);
+}
+
 // nearest_n / nearest_n().within for k above the register forms (k > 64): the top-k is a
 // max-heap in the query's output row (device.h RowHeap), sorted in place at the end.
 // SCREEN: f64 trees with the FP32 shadow (SquaredEuclidean only).

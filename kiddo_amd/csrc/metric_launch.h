@@ -300,6 +300,20 @@ int mlaunch_nearest_n(const QueryArgs& a, M met, int k, double radius, uint32_t*
   return KD_OK;
 }
 
+// Lanes per query of best_n_within (SquaredEuclidean): the tree's setting
+// (kd_tree_tune_best_n_within; a group smaller than n means one thread), else
+// automatically 16 lanes below 2^17 queries, 8 (n <= 8) or 16 below 2^22, one thread above.
+// The family scans every entry in range, which the lanes split. Measured (best_n_within(5),
+// r = 0.001, clustered 12M-point f64 tree, ~480 entries in range per query): 16 / 8 lanes /
+// one thread 0.34 / 0.45 / 1.56 ms at 1k queries, 0.64 / 0.76 / 2.03 at 64k, 1.39 / 1.17 /
+// 4.00 at 256k, 4.45 / 3.16 / 6.69 at 1M, 15.0 / 10.6 / 13.2 at 4M.
+inline int best_lanes(const kd_tree* t, uint64_t m, int n) {
+  if (t->best_group == 1) return 1;
+  if (t->best_group > 1) return t->best_group >= n ? t->best_group : 1;
+  if (n > 16 || m >= (uint64_t(1) << 22)) return 1;
+  return (n > 8 || m < (uint64_t(1) << 17)) ? 16 : 8;
+}
+
 template <class A, int K, int MAXK, class M>
 int mlaunch_best_typed(const QueryArgs& a, M met, int n, double radius, uint32_t* items, void* dists,
                        uint32_t* counts) {
@@ -307,6 +321,23 @@ int mlaunch_best_typed(const QueryArgs& a, M met, int n, double radius, uint32_t
   auto dt = dev_tree<A, K>(a.t);
   reset_counters(a.t, a.s);
   KernelTimer kt(a.t, a.s);
+  if constexpr (std::is_same<M, dev::SqEuclidean>::value) {
+    auto group = [&](auto gc) -> int {
+      constexpr int G = decltype(gc)::value;
+      hipLaunchKernelGGL((dev::k_best_n_within_group<A, K, G>), dim3(grid_for(a.m * G)), dim3(dev::kBlock),
+                         lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, n, O(radius),
+                         (a.flags & KD_EXCLUSIVE) ? 1 : 0, items, static_cast<O*>(dists), counts, a.t->counters);
+      KD_HIP(hipGetLastError());
+      return KD_OK;
+    };
+    switch (best_lanes(a.t, a.m, n)) {
+      case 4: return group(std::integral_constant<int, 4>{});
+      case 8: return group(std::integral_constant<int, 8>{});
+      case 16: return group(std::integral_constant<int, 16>{});
+      case 32: return group(std::integral_constant<int, 32>{});
+      default: break;
+    }
+  }
   hipLaunchKernelGGL((dev::k_best_n_within<A, K, MAXK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock),
                      lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, n, O(radius),
                      (a.flags & KD_EXCLUSIVE) ? 1 : 0, items, static_cast<O*>(dists), counts, a.t->counters);

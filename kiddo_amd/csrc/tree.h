@@ -39,6 +39,7 @@ struct kd_tree {
   int waves = 0;        // minimum waves per SIMD the k-NN kernels are compiled for (0 = 3)
   int fill_group = 0;   // within fill: 0 automatic, 1 thread per query, else lanes per query
   int count_group = 0;  // within count: 0 or 1 thread per query, else lanes per query
+  int best_group = 0;   // best_n_within: 0 automatic, 1 thread per query, else lanes per query
   // optional HIP-event timing of the main query kernel of each call
   bool timing = false;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;

@@ -176,8 +176,9 @@ def test_screen_four_dims(ka, oracle_mod, cuda, kind):
                 r = t.nearest_n_raw(q, k, radius, f)
                 _cmp((r.items.cpu().numpy(), r.distances.cpu().numpy(), r.counts.cpu().numpy()), ref,
                      f"{kind} k={k} r={radius} excl={excl} variant={variant} flags={f}")
-    t.tune(0, 0, -1)
+    t.tune(1, 0, -1)  # the FP32 collector (this batch would take the group form automatically)
     t.nearest_n_raw(q, 16, flags=ka.KD_SCREEN_F32)
+    t.tune(0, 0, -1)
     st = t.query_stats()
     redos = t.query_redos()
     assert st["rescored"] <= st["points"], st

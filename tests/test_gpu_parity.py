@@ -373,6 +373,9 @@ def test_screened_redo_pass(ka, oracle_mod, cuda):
         q = rng.random((4000, 3)) * float(pts.max())
         q[:1000] = pts[rng.integers(0, len(pts), 1000)]
         t = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
+        # one thread per query: the FP32 collector and its redo pass (a batch this small
+        # would take the group form automatically)
+        t.tune(1, 0, -1)
         O = OracleEngine(oracle_mod)
         ot = O.build(pts, 32)
         for k in (1, 4, 8, 16):
@@ -391,6 +394,7 @@ def test_screened_redo_pass(ka, oracle_mod, cuda):
     # no ties: nothing is redone
     u = rng.random((20_000, 3))
     tu = ka.ImmutableKdTree.new_from_slice(u, 32, shadow_f32=True)
+    tu.tune(1, 0, -1)
     tu.nearest_n_raw(rng.random((4000, 3)), 16, math.inf, ka.KD_SCREEN_F32 | ka.KD_MORTON)
     assert tu.query_redos() == 0
 
@@ -403,6 +407,7 @@ def test_screened_redo_concurrent_streams(ka, oracle_mod, cuda):
     rng = np.random.default_rng(11)
     pts = rng.integers(0, 7, (40_000, 3)).astype(np.float64)
     t = ka.ImmutableKdTree.new_from_slice(pts, 32, shadow_f32=True)
+    t.tune(1, 0, -1)  # one thread per query: the FP32 collector and its redo pass
     O = OracleEngine(oracle_mod)
     ot = O.build(pts, 32)
     qs = [rng.random((30_000, 3)) * 6.0 for _ in range(2)]
