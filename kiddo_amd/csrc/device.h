@@ -1778,6 +1778,48 @@ template <class O, int G> struct GroupWithinStageVis {
   }
 };
 
+// Group nearest_one collector (nearest_one.rs:330-352, canonical rule as NearestOneVis):
+// the G lanes of a query walk group_traverse together and measure G entries of a leaf per
+// step; the step's smallest (distance, item) is found by a butterfly over the group and
+// replaces the best when it is smaller in that order. The thread collector's result is the
+// smallest (distance, item) of (+inf, 0) and every entry it measures, which does not depend
+// on the order the entries are seen in, so both return the same.
+template <class O, int G> struct GroupNearestOneVis {
+  O best;
+  uint32_t item;
+  int gl;
+  __device__ O bound() const { return best; }
+  __device__ bool prune(O rd) const { return rd > best; }
+  __device__ bool prune_at(O rd, O b) const { return rd > b; }
+  __device__ bool unbounded() const { return true; }
+  template <class A, int K, class QW>
+  __device__ void leaf(const DevTree<A, K>& t, const QW& qw, uint32_t leaf) {
+    const uint32_t s = t.leaf_start[leaf], e = t.leaf_start[leaf + 1];
+    for (uint32_t p0 = s; p0 < e; p0 += G) {
+      const uint32_t p = p0 + uint32_t(gl);
+      O d = oinf<O>();
+      uint32_t it = 0xFFFFFFFFu;
+      if (p < e) {
+        d = point_dist(t, p, qw);
+        it = t.items[p];
+      }
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) {
+        const O od = __shfl_xor(d, o, G);
+        const uint32_t oi = __shfl_xor(it, o, G);
+        if (od < d || (od == d && oi < it)) {
+          d = od;
+          it = oi;
+        }
+      }
+      if (d < best || (d == best && it < item)) {
+        best = d;
+        item = it;
+      }
+    }
+  }
+};
+
 // Group best_n_within collector (n <= G; best_n_within.rs:163-196): the G lanes of a query
 // run its traversal together (the radius is the only bound, so they take one path) and
 // split each leaf G entries at a time. Lane gl keeps slot gl of the n smallest items in

@@ -31,6 +31,36 @@ __global__ __launch_bounds__(kBlock) void k_nearest_one(DevTree<A, K> t, const A
   flush_counters(cnt, stats);
 }
 
+// nearest_one, G lanes per query (SquaredEuclidean; GroupNearestOneVis): for batches too
+// small to fill the device one thread per query. Lane 0 of the group writes the result.
+template <class A, int K, int G>
+__global__ __launch_bounds__(kBlock) void k_nearest_one_group(DevTree<A, K> t, const A* __restrict__ queries,
+                                                              const uint32_t* __restrict__ order, uint64_t m,
+                                                              uint32_t* __restrict__ out_items,
+                                                              typename Ax<A>::O* __restrict__ out_d,
+                                                              unsigned long long* __restrict__ stats) {
+  using O = typename Ax<A>::O;
+  pin_stems(t);
+  const uint64_t gid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / G;
+  const int gl = int(threadIdx.x) & (G - 1);
+  const bool active = gid < m;  // uniform within a group
+  Counters c;
+  if (active) {
+    const uint64_t qi = order ? order[gid] : gid;
+    Small<A, K> q;
+    Small<O, K> qw;
+    load_query<A, K>(queries, qi, q, qw);
+    GroupNearestOneVis<O, G> v{oinf<O>(), 0u, gl};
+    group_traverse<A, K, G>(t, q, qw, v, c, gl);
+    if (gl == 0) {
+      out_items[qi] = v.item;
+      out_d[qi] = v.best;
+    }
+    if (gl != 0) c = Counters{};  // counted once per group
+  }
+  flush_counters(c, stats);
+}
+
 template <class A, int K, class M>
 __global__ __launch_bounds__(kBlock) void k_approx_nearest_one(DevTree<A, K> t, const A* __restrict__ queries,
                                                                const uint32_t* __restrict__ order, uint64_t m,

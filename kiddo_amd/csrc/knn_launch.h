@@ -174,17 +174,6 @@ __global__ __launch_bounds__(kBlock) void k_knn_group(DevTree<A, K> t, const A* 
 
 namespace knn {
 
-// Automatic k-NN form (knn_variant 0) for k <= 16: the group form for batches too small to
-// keep the device busy one thread per query, whose kernel time is then one query's chain
-// of dependent loads. Measured (nearest_n(5), 12M-point f64 tree: 8 lanes 0.12 / 0.38 /
-// 0.69 ms against 0.28 / 0.67 / 1.18 ms one thread per query at 1k / 64k / 256k queries,
-// 2.17 against 1.77 ms at 1M; nearest_n(8) on a 100k-point tree, which stays in the L2,
-// at 100k queries: 0.22 against 0.18 ms): below 2^16 queries, or 2^19 on trees of at
-// least 2^22 entries.
-inline bool knn_small_batch(const kd_tree* t, uint64_t m) {
-  return m < (uint64_t(1) << 16) || (m < (uint64_t(1) << 19) && t->size >= (uint64_t(1) << 22));
-}
-
 template <class A, int K, int MAXK, int WAVES>
 int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts);
 
@@ -286,9 +275,9 @@ int launch_typed(const QueryArgs& a, int k, double radius, uint32_t* items, void
     return mlaunch_knn_heap<A, K, dev::SqEuclidean, false>(a, {}, k, radius, items, dists, counts);
   }
   // group form: k must fit in one slot per lane. Automatic (knn_variant 0): small batches
-  // (knn_small_batch) with k <= 16 take 8 lanes per query (k <= 8) or 16.
+  // (small_batch) with k <= 16 take 8 lanes per query (k <= 8) or 16.
   int g = t->group_size;
-  const bool small = t->knn_variant == 0 && k <= 16 && knn_small_batch(t, a.m);
+  const bool small = t->knn_variant == 0 && k <= 16 && small_batch(t, a.m);
   if (g == 0) g = k <= 4 ? (small ? 8 : 4) : k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
   const bool group = (t->knn_variant == 2 || small) && g >= k;
   if (group) {

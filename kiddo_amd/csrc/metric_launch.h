@@ -228,12 +228,33 @@ __global__ __launch_bounds__(kBlock) void k_periodic_within_fill(DevTree<A, K> t
 
 // ------------------------------------------------------------------ launchers --------
 
+// Batches too small to keep the device busy one thread per query: the kernel time is then
+// one query's chain of dependent loads, and nearest_one and nearest_n (k <= 16) take their
+// group forms when the tree's k-NN form is automatic (knn_variant 0). Measured
+// (nearest_n(5), 12M-point f64 tree: 8 lanes 0.12 / 0.38 / 0.69 ms against 0.28 / 0.67 /
+// 1.18 ms one thread per query at 1k / 64k / 256k queries, 2.17 against 1.77 ms at 1M;
+// nearest_n(8) on a 100k-point tree, which stays in the L2, at 100k queries: 0.22 against
+// 0.18 ms): below 2^16 queries, or 2^19 on trees of at least 2^22 entries.
+inline bool small_batch(const kd_tree* t, uint64_t m) {
+  return m < (uint64_t(1) << 16) || (m < (uint64_t(1) << 19) && t->size >= (uint64_t(1) << 22));
+}
+
 template <class M> int mlaunch_nearest_one(const QueryArgs& a, M met, uint32_t* items, void* dists) {
   KD_DISPATCH_AK(a.t->dtype, a.t->dims, {
     using O = typename dev::Ax<A>::O;
     auto dt = dev_tree<A, KK>(a.t);
     reset_counters(a.t, a.s);
     KernelTimer kt(a.t, a.s);
+    if constexpr (std::is_same<M, dev::SqEuclidean>::value) {
+      if (a.t->knn_variant == 0 && small_batch(a.t, a.m)) {
+        constexpr int G = 8;
+        hipLaunchKernelGGL((dev::k_nearest_one_group<A, KK, G>), dim3(grid_for(a.m * G)), dim3(dev::kBlock),
+                           lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, items,
+                           static_cast<O*>(dists), a.t->counters);
+        KD_HIP(hipGetLastError());
+        return KD_OK;
+      }
+    }
     hipLaunchKernelGGL((dev::k_nearest_one<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds_bytes_for(dt),
                        a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, items, static_cast<O*>(dists),
                        a.t->counters);

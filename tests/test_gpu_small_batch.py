@@ -3,8 +3,9 @@ bits, counts):
   * best_n_within one thread per query and 4/8/16/32 lanes per query, and the automatic
     choice, on the parity cases (n from 1 to 20, inclusive and exclusive radius);
   * with repeated items, every shape keeps the same entries as one thread per query;
-  * nearest_n's automatic form, which takes the group kernels on small batches (below 2^16
-    queries, or 2^19 on trees of at least 2^22 entries), plain and FP32-screened.
+  * nearest_one's and nearest_n's automatic forms, which take the group kernels on small
+    batches (below 2^16 queries, or 2^19 on trees of at least 2^22 entries), plain and
+    FP32-screened, and nearest_one where every FP32 distance overflows to +inf.
 """
 import math
 
@@ -97,6 +98,13 @@ def test_nearest_n_automatic_small_batch(ka, oracle_mod, cuda, kind, n, K, dtype
         gt = G.build(pts, B)
         for m in (1, 100, 3000):
             q = _queries(pts, m, seed=m + n)
+            ref1 = O.nearest_one(ot, q)
+            for variant in (0, 1):  # automatic (the group form here) and one thread per query
+                gt.tune(variant, 0, -1)
+                res1 = G.nearest_one(gt, q)
+                for a, b, nm in zip(res1, ref1, ("items", "dists")):
+                    assert_same_bits(a, b, f"nearest_one {nm} m={m} variant={variant} flags={G.flags}")
+            gt.tune(0, 0, -1)
             for k in (1, 4, 5, 8, 9, 16):
                 for radius, excl in ((math.inf, False), (r, False), (r, True)):
                     ref = O.nearest_n(ot, q, k, radius, excl)
@@ -126,3 +134,23 @@ def test_nearest_n_automatic_large_tree(ka, oracle_mod, cuda):
         for a, b, c, nm in zip(res, ref, res_t, NAMES):
             assert_same_bits(a, b, f"nearest_n({k}) {nm} automatic")
             assert_same_bits(c, b, f"nearest_n({k}) {nm} thread per query")
+
+
+def test_nearest_one_infinite_distances(ka, oracle_mod, cuda):
+    """f32 coordinates near 1e20 square to +inf: every distance is +inf, and the group and
+    thread forms return what the oracle returns."""
+    _announce("nearest_one infinite distances")
+    rng = np.random.default_rng(5)
+    pts = (rng.random((5000, 3)) * 1e20 + 1e20).astype(np.float32)
+    q = (-rng.random((700, 3)) * 1e20 - 1e20).astype(np.float32)
+    O = OracleEngine(oracle_mod)
+    ot = O.build(pts, 32)
+    ref = O.nearest_one(ot, q)
+    assert np.all(np.isinf(ref[1]))
+    G = GpuEngine(ka, flags=0)
+    gt = G.build(pts, 32)
+    for variant in (0, 1):
+        gt.tune(variant, 0, -1)
+        res = G.nearest_one(gt, q)
+        for a, b, nm in zip(res, ref, ("items", "dists")):
+            assert_same_bits(a, b, f"nearest_one inf {nm} variant={variant}")
