@@ -155,7 +155,8 @@ int kd_tree_get_info(const kd_tree* tree, kd_tree_info* out);
 /* Launch tuning (no Kiddo counterpart; results never depend on it).
  * knn_variant: 0 automatic (for k <= 16, group_size lanes per query, 0 = 8 or 16, on
  * batches below 2^16 queries, or 2^19 on trees of at least 2^22 entries; else one thread
- * per query), 1 one thread per query, 2 one group of group_size lanes per query
+ * per query; nearest_one takes 8 lanes per query on the same batches), 1 one thread per
+ * query (nearest_one too), 2 one group of group_size lanes per query
  * (group_size 0 = smallest of 4/8/16/32/64 that holds k). lds_levels: top stem
  * levels copied into LDS per block (-1 automatic, 0 none, at most 14). */
 int kd_tree_tune(kd_tree* tree, int32_t knn_variant, int32_t group_size, int32_t lds_levels);
