@@ -82,7 +82,7 @@ def main():
                     help="serving: requests of at least this many queries are Morton-ordered")
     ap.add_argument("--max-part", type=int, default=16384,
                     help="serving: requests are cut into parts of at most this many queries (0: never)")
-    ap.add_argument("--rates", default="2000,8000,16000,32000",
+    ap.add_argument("--rates", default="2000,8000,16000,32000,64000",
                     help="serving: arrival rates (per second) of the rate sweep before the main run")
     ap.add_argument("--trees", type=int, default=4096)
     ap.add_argument("--forest-bucket", type=int, default=0,

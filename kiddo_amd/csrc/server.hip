@@ -9,13 +9,15 @@
 //   * coalesces the admitted part with other pending parts of the same kind (same tree,
 //     family, k, radius and flags), at most one part per request, into one launch of up to
 //     `max_coalesce` queries;
-//   * runs each launch on its slot's stream: host -> pinned staging -> HBM, the query kernel
-//     through the C ABI, HBM -> pinned staging, with an event at the end;
-//   * retires finished launches by polling their events, copies every part's slice of the
-//     results into its request's host buffers, frees the arena range and the slot, and stamps
-//     a request's latency when its last part is answered.
-// Launches on different slots overlap on the device; a request's results do not depend on
-// what it was batched with, because every query is answered independently.
+//   * hands each launch to its slot's worker thread.
+// A slot's worker runs the launch on the slot's stream (host -> pinned staging -> HBM, the
+// query kernel through the C ABI, HBM -> pinned staging), waits for it, copies every part's
+// slice of the results into its request's host buffers, frees the arena range and the slot,
+// and stamps a request's latency when its last part is answered. The workers' staging
+// copies, launches and result copies run side by side (one loop thread doing all of them
+// kept the device idle about half of the time at saturation). Launches on different slots
+// overlap on the device; a request's results do not depend on what it was batched with,
+// because every query is answered independently.
 
 #include <hip/hip_runtime.h>
 
@@ -26,6 +28,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -44,10 +47,9 @@ struct Req {
   double latency_ms = 0;
   bool done = false;
   int status = KD_OK;
-  // loop thread only
-  uint32_t parts_left = 0;  // parts not answered yet
-  uint32_t in_flight = 0;   // parts on the device
-  uint64_t launch_mark = 0; // the launch being assembled holds one of its parts
+  uint32_t parts_left = 0;             // parts not answered yet (under the server's mutex)
+  std::atomic<uint32_t> in_flight{0};  // parts on the device (read by the loop thread)
+  uint64_t launch_mark = 0;            // loop thread only: the launch being assembled holds one of its parts
 };
 
 // Queries [first, first + m) of a request: the unit of admission.
@@ -112,6 +114,15 @@ struct Launch {
   hipEvent_t done_ev = nullptr;
 };
 
+// The worker thread of one stream slot and the launch handed to it.
+struct SlotWorker {
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  bool has = false, stop = false;
+  Launch L;
+};
+
 }  // namespace
 }  // namespace kd
 
@@ -126,6 +137,8 @@ struct kd_server {
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
   std::vector<bool> slot_busy;
+  std::vector<std::unique_ptr<kd::SlotWorker>> workers;
+  bool freed = false;  // a slot or arena range was released since the loop last looked
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<kd::Part*> submitted;
@@ -197,65 +210,92 @@ int run_launch(kd_server* S, Launch& L) {
   return KD_OK;
 }
 
+// Runs on the slot's worker thread. Parts of one request may finish on several workers at
+// once: each copies its own slice, then the request's counters change under the mutex, and
+// the worker whose part was the last one marks the request done (its waiter may delete it
+// as soon as the mutex is released).
 void finish_launch(kd_server* S, Launch& L, int status) {
-  const auto now = Clock::now();
-  std::vector<Req*> finished;
-  for (size_t i = 0; i < L.parts.size(); ++i) {
-    Part* P = L.parts[i];
-    Req* R = P->req;
-    const uint64_t b = L.q_begin[i], m = P->m, f = P->first;
-    if (status == KD_OK) {
-      if (R->r.items) std::memcpy(R->r.items + f * L.k, S->pinned + L.items_off + b * L.k * 4, m * L.k * 4);
-      if (R->r.dists)
-        std::memcpy(static_cast<uint8_t*>(R->r.dists) + f * L.k * L.o_size,
+  if (status == KD_OK) {
+    for (size_t i = 0; i < L.parts.size(); ++i) {
+      const Part* P = L.parts[i];
+      const kd_request& r = P->req->r;
+      const uint64_t b = L.q_begin[i], m = P->m, f = P->first;
+      if (r.items) std::memcpy(r.items + f * L.k, S->pinned + L.items_off + b * L.k * 4, m * L.k * 4);
+      if (r.dists)
+        std::memcpy(static_cast<uint8_t*>(r.dists) + f * L.k * L.o_size,
                     S->pinned + L.dists_off + b * L.k * L.o_size, m * L.k * L.o_size);
-      if (R->r.counts && L.k > 1 && R->r.family != KD_FAMILY_NEAREST_ONE)
-        std::memcpy(R->r.counts + f, S->pinned + L.counts_off + b * 4, m * 4);
-    } else if (R->status == KD_OK) {
-      R->status = status;
+      if (r.counts && L.k > 1 && r.family != KD_FAMILY_NEAREST_ONE)
+        std::memcpy(r.counts + f, S->pinned + L.counts_off + b * 4, m * 4);
     }
+  }
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> g(S->mu);
+  uint64_t finished = 0;
+  for (Part* P : L.parts) {
+    Req* R = P->req;
+    if (status != KD_OK && R->status == KD_OK) R->status = status;
     --R->in_flight;
     if (--R->parts_left == 0) {
       R->latency_ms = std::chrono::duration<double, std::milli>(now - R->submitted).count();
-      finished.push_back(R);
+      R->done = true;
+      ++finished;
     }
     delete P;
   }
-  std::lock_guard<std::mutex> g(S->mu);
+  L.parts.clear();
   S->arena.release(L.off, L.bytes);
   S->slot_busy[L.slot] = false;
-  for (Req* R : finished) R->done = true;  // R may be deleted by its waiter from here on
-  S->st_done += finished.size();
+  S->freed = true;
+  S->st_done += finished;
   S->cv_done.notify_all();
+  S->cv_work.notify_one();
 }
 
+// One stream slot: runs the launches the loop thread hands it, one at a time.
+void slot_worker(kd_server* S, uint32_t j) {
+  (void)hipSetDevice(S->device);
+  SlotWorker& W = *S->workers[j];
+  for (;;) {
+    Launch L;
+    {
+      std::unique_lock<std::mutex> g(W.mu);
+      W.cv.wait(g, [&] { return W.has || W.stop; });
+      if (!W.has) return;
+      L = std::move(W.L);
+      W.has = false;
+    }
+    int rc = run_launch(S, L);
+    if (rc == KD_OK && hipEventSynchronize(L.done_ev) != hipSuccess)
+      rc = set_error(KD_ERR_DEVICE, "a served launch failed on the device");
+    // after a failure, the stream's queued copies must end before the arena range is reused
+    if (rc != KD_OK) (void)hipStreamSynchronize(S->streams[L.slot]);
+    finish_launch(S, L, rc);
+  }
+}
+
+// The loop thread: admission and coalescing; launches run on the slot workers.
 void serve(kd_server* S) {
   (void)hipSetDevice(S->device);
-  std::vector<Launch> inflight;
   std::deque<Part*> pending;
   uint64_t serial = 0;
+  bool progressed = true;
   for (;;) {
     {
       std::unique_lock<std::mutex> g(S->mu);
-      if (pending.empty() && inflight.empty() && S->submitted.empty())
-        S->cv_work.wait_for(g, std::chrono::microseconds(200), [&] { return S->stop || !S->submitted.empty(); });
-      if (S->stop && pending.empty() && inflight.empty() && S->submitted.empty()) return;
+      // sleep until a request arrives or a worker frees a slot or memory (or 200 us pass)
+      if (!progressed)
+        S->cv_work.wait_for(g, std::chrono::microseconds(200),
+                            [&] { return S->stop || !S->submitted.empty() || S->freed; });
+      S->freed = false;
+      const bool idle = std::none_of(S->slot_busy.begin(), S->slot_busy.end(), [](bool b) { return b; });
+      if (S->stop && pending.empty() && S->submitted.empty() && idle) return;
       while (!S->submitted.empty()) {
         pending.push_back(S->submitted.front());
         S->submitted.pop_front();
       }
     }
-    // retire finished launches
-    for (size_t i = 0; i < inflight.size();) {
-      if (hipEventQuery(inflight[i].done_ev) == hipSuccess) {
-        finish_launch(S, inflight[i], KD_OK);
-        inflight.erase(inflight.begin() + i);
-      } else {
-        ++i;
-      }
-    }
     // admit in arrival order while a slot and memory are free
-    bool progressed = false;
+    progressed = false;
     while (!pending.empty()) {
       int slot = -1;
       {
@@ -323,15 +363,15 @@ void serve(kd_server* S) {
       ++S->st_launches;
       S->st_queries += L.m;
       S->st_coalesced += L.parts.size() - 1;
-      const int rc = run_launch(S, L);
-      if (rc) {
-        finish_launch(S, L, rc);
-      } else {
-        inflight.push_back(L);
+      {
+        SlotWorker& W = *S->workers[slot];
+        std::lock_guard<std::mutex> g(W.mu);
+        W.L = std::move(L);
+        W.has = true;
       }
+      S->workers[slot]->cv.notify_one();
       progressed = true;
     }
-    if (!progressed && !inflight.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
 }
 
@@ -378,6 +418,8 @@ int kd_server_create(kd_tree* const* trees, uint32_t ntrees, uint64_t arena_byte
     delete S;
     return rc;
   }
+  for (uint32_t j = 0; j < slots; ++j) S->workers.push_back(std::make_unique<kd::SlotWorker>());
+  for (uint32_t j = 0; j < slots; ++j) S->workers[j]->th = std::thread(kd::slot_worker, S, j);
   S->loop = std::thread(kd::serve, S);
   *out = S;
   return KD_OK;
@@ -452,7 +494,15 @@ int kd_server_destroy(kd_server* S) {
     S->stop = true;
   }
   S->cv_work.notify_all();
-  if (S->loop.joinable()) S->loop.join();
+  if (S->loop.joinable()) S->loop.join();  // returns once every admitted launch has finished
+  for (auto& W : S->workers) {
+    {
+      std::lock_guard<std::mutex> g(W->mu);
+      W->stop = true;
+    }
+    W->cv.notify_one();
+    if (W->th.joinable()) W->th.join();
+  }
   for (auto s : S->streams) (void)hipStreamDestroy(s);
   for (auto e : S->events) (void)hipEventDestroy(e);
   (void)hipFree(S->dev);

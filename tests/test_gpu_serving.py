@@ -163,3 +163,44 @@ def test_serving_parts_equal_alone_and_oracle(ka, serving_case):
             _same(b, o.nearest_n(q, k, r), f"req {i} nearest_n")
         else:
             _same(b, o.best_n_within(q, r, k), f"req {i} best_n_within")
+
+
+def test_serving_slot_workers_stress(ka, serving_case):
+    """8 slots (one worker thread each), parts of 250 queries, 240 requests from 4 client
+    threads: parts of one request finish on different workers in any order, and every
+    request equals the same request on a one-slot server, bit for bit."""
+    from kiddo_amd.datasets import latlon_clusters, palette_u16
+    trees, _, _ = serving_case
+    rng = np.random.default_rng(23)
+    reqs = []
+    for i in range(240):
+        tree = int(rng.integers(0, 2))
+        fam = [ka.FAMILY_NEAREST_ONE, ka.FAMILY_NEAREST_N, ka.FAMILY_BEST_N_WITHIN][i % 3]
+        m = int(rng.integers(1, 3000))
+        q = latlon_clusters(m, seed=3000 + i) if tree == 0 else palette_u16(m, seed=4000 + i)
+        r = (1e-4 if tree == 0 else 4.0e6) if fam == ka.FAMILY_BEST_N_WITHIN else np.inf
+        reqs.append((tree, fam, q, 5, r))
+    srv1 = ka.Server(trees, arena_bytes=64 << 20, slots=1, max_coalesce=1, max_part=0)
+    alone = _run_alone(ka, srv1, reqs)
+    srv1.close()
+    srv = ka.Server(trees, arena_bytes=64 << 20, slots=8, max_coalesce=2000, max_part=250)
+    tickets = [None] * len(reqs)
+
+    def client(idx):
+        for i in idx:
+            tree, fam, q, k, r = reqs[i]
+            tickets[i] = srv.submit(tree, fam, q, k, r)
+
+    order = rng.permutation(len(reqs))
+    th = [threading.Thread(target=client, args=(order[j::4],)) for j in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    mixed = [srv.wait(tk) for tk in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["done"] == len(reqs), st
+    for i, (a, b) in enumerate(zip(alone, mixed)):
+        _same(a, b, f"request {i} on 8 slot workers vs alone")
+        assert b[3] > 0.0
