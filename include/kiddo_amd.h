@@ -359,9 +359,11 @@ int kd_forest_autotune_bucket(int32_t dtype, int32_t dims, uint32_t ntrees, cons
  * host memory) has room for its queries and results, taking first the oldest part of a
  * request with the fewest parts on the device (a large request does not hold every slot
  * while smaller ones wait); it coalesces pending parts of the same kind (tree, family, k,
- * radius, flags), one per request, into the same launch, up to `max_coalesce` queries;
- * launches on different slots overlap on the device. A request completes when its last
- * part does; results never depend on the batching. */
+ * radius, flags), one per request, into the same launch, up to `max_coalesce` queries.
+ * Each slot has a worker thread that stages, runs and copies back its launches, so the
+ * server runs 1 + `slots` threads; launches on different slots overlap on the device. A
+ * request completes when its last part does; results never depend on the batching.
+ * Served trees should not have timing enabled (the slots' calls share the tree's events). */
 enum kd_family {
   KD_FAMILY_NEAREST_ONE = 1,   /* items[m], dists[m]                                   */
   KD_FAMILY_NEAREST_N = 2,     /* items[m*k], dists[m*k], counts[m]; radius as kd_nearest_n */
@@ -374,7 +376,7 @@ typedef struct kd_request {
   uint32_t k;
   uint32_t flags;     /* KD_EXCLUSIVE, KD_MORTON, KD_SCREEN_F32 */
   double radius;      /* +INFINITY for none */
-  const void* queries; /* host, m x dims of the tree's dtype; read during kd_server_submit's admission */
+  const void* queries; /* host, m x dims of the tree's dtype; read when a part is launched */
   uint64_t m;
   uint32_t* items;    /* host outputs, valid once kd_server_wait returns */
   void* dists;
