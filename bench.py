@@ -692,6 +692,28 @@ def _time_knn(tree, q, k, flags, steps, warmup, torch, out=None):
     return (time.perf_counter() - t0) / steps, float(np.mean(kms)), out
 
 
+def _row_roofline(st, m, k, K, esz, osz, kms, kernel, traffic=None):
+    """HBM roofline of one k-row query launch from the tree's visit counters (the last call's
+    query_stats): every entry scanned (K coordinates + u32 item), every stem read, the
+    queries, and the rows written (k x (u32 item + distance)); achieved = those bytes / the
+    kernel's HIP-event time."""
+    algo = st["points"] * (K * esz + 4) + st["stems"] * esz + m * K * esz + m * k * (4 + osz)
+    ach = algo / (kms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+            "kernel_ms": round(kms, 3), "algo_bytes_per_launch": int(algo)}
+
+
+def _traffic(name):
+    """Measured HBM bytes per launch by kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE summaries
+    committed under profiles/), or {} when absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh)
+
+
 def bench_mixture(args, torch, ka):
     """ImmutableKdTree<f32, 3>: 10,000,000 points from a 64-component isotropic Gaussian
     mixture in the unit cube (centres uniform, sigma 0.01, clipped), bucket 32; 10,000,000
@@ -824,6 +846,8 @@ def bench_small(args, torch, ka):
     for name, k in (("nearest_n8", 8),):
         dt, kms, _ = _time_knn(tree, q, k, ka.KD_MORTON, max(args.steps, 20), args.warmup, torch)
         res[name] = {"ms": round(dt * 1e3, 4), "kernel_ms": round(kms, 4), "qps": round(m / dt, 1)}
+        # the 100k-point tree (2.8 MB) stays in the L2: no HBM traffic figure
+        roof = _row_roofline(tree.query_stats(), m, k, 3, 8, 8, kms, "k_nearest_n<double,3,8>")
     for _ in range(args.warmup):
         tree.nearest_one_raw(q, ka.KD_MORTON)
     torch.cuda.synchronize()
@@ -841,7 +865,7 @@ def bench_small(args, torch, ka):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded uniform)", "config": {"workload": "small-100k-f64", "points": n,
                                                          "queries": m},
-        "families": res,
+        "families": res, "roofline": roof,
     }), flush=True)
 
 
@@ -876,6 +900,9 @@ def bench_periodic(args, torch, ka):
     st = tree.query_stats()
     res[f"periodic_nearest_n{k}"] = {"ms": round(dt * 1e3, 2), "kernel_ms": round(kms, 2),
                                     "qps": round(m / dt, 1), "leaves_per_query": round(st["leaves"] / m, 2)}
+    kname = f"k_periodic_nearest_n<double,3,{k}>"
+    roof = _row_roofline(st, m, k, 3, 8, 8, kms, kname,
+                         _traffic("periodic_16M_f64_traffic.json").get(kname, {}).get("bytes_per_launch"))
     dtc, kmsc = timed(lambda: tree.nearest_n_raw(q, k, flags=M), args.steps)
     res[f"cartesian_nearest_n{k}"] = {"ms": round(dtc * 1e3, 2), "kernel_ms": round(kmsc, 2),
                                      "qps": round(m / dtc, 1)}
@@ -913,7 +940,7 @@ def bench_periodic(args, torch, ka):
         "data": "synthetic (seeded uniform in the unit box)",
         "config": {"workload": f"periodic-f64-3d-{n}-{m}q", "points": n, "queries": m, "k": k,
                    "box": [1.0, 1.0, 1.0]},
-        "families": res, "cpu_baseline": cpu,
+        "families": res, "roofline": roof, "cpu_baseline": cpu,
     }), flush=True)
 
 
@@ -931,6 +958,7 @@ def bench_metrics(args, torch, ka):
     q = torch.from_numpy(qh).cuda()
     flags = {"sqeuclidean": 0, "manhattan": ka.KD_METRIC_MANHATTAN, "chebyshev": ka.KD_METRIC_CHEBYSHEV,
              "minkowski3": ka.KD_METRIC_MINKOWSKI(3)}
+    tr = _traffic("metrics_16M_f64_traffic.json")
     res = {}
     for name, f in flags.items():
         for _ in range(args.warmup):
@@ -945,9 +973,12 @@ def bench_metrics(args, torch, ka):
         dt = (time.perf_counter() - t0) / args.steps
         st = tree.query_stats()
         kth = float(out.distances[:, -1].float().median().item())
+        kname = f"k_nearest_n<double,3,{k},{name}>"
         res[name] = {"nearest_n_ms": round(dt * 1e3, 2), "kernel_ms": round(float(np.mean(kms)), 2),
                      "qps": round(m / dt, 1), "leaves_per_query": round(st["leaves"] / m, 2),
-                     "median_kth_distance": kth}
+                     "median_kth_distance": kth,
+                     "roofline": _row_roofline(st, m, k, 3, 8, 8, float(np.mean(kms)), kname,
+                                               tr.get(kname, {}).get("bytes_per_launch"))}
         t0 = time.perf_counter()
         w = tree.within_raw(q, kth, f | ka.KD_MORTON)
         torch.cuda.synchronize()
@@ -978,7 +1009,7 @@ def bench_metrics(args, torch, ka):
         "warmup": args.warmup, "ms_per_step": res["manhattan"]["nearest_n_ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform)",
         "config": {"workload": f"metrics-f64-3d-{n}-{m}q", "points": n, "queries": m, "k": k},
-        "by_metric": res, "cpu_baseline": cpu,
+        "by_metric": res, "roofline": res["manhattan"]["roofline"], "cpu_baseline": cpu,
     }), flush=True)
 
 
