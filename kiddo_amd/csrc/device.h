@@ -1992,8 +1992,31 @@ template <class A, int K, int G, bool SCREEN> struct GroupKnnVis {
   }
 };
 
-// Sorts one query's segment by (distance, item): insertion sort for short segments,
-// heapsort otherwise (in place, global memory).
+// Sorts a segment of at most N entries in registers (Batcher's network over N slots, the
+// slots past len padded with (+inf, ~0u), which sort last): one read and one write of each
+// entry.
+template <class O, int N>
+__device__ __forceinline__ void sort_segment_regs(uint32_t* __restrict__ it, O* __restrict__ d, uint32_t len) {
+  TopK<O, N> r;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const uint32_t src = uint32_t(j) < len ? uint32_t(j) : 0u;  // in the segment, len >= 1
+    const O dv = d[src];
+    const uint32_t iv = it[src];
+    r.d[j] = uint32_t(j) < len ? dv : oinf<O>();
+    r.it[j] = uint32_t(j) < len ? iv : 0xFFFFFFFFu;
+  }
+  r.sort();
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    if (uint32_t(j) < len) {
+      d[j] = r.d[j];
+      it[j] = r.it[j];
+    }
+}
+
+// Sorts one query's segment by (distance, item), in place in global memory: insertion sort
+// up to 48 entries, heapsort above. Few registers, for use inside a query kernel.
 template <class O>
 __device__ void sort_segment(uint32_t* __restrict__ it, O* __restrict__ d, uint64_t len) {
   auto less = [](O ad, uint32_t ai, O bd, uint32_t bi) { return ad < bd || (ad == bd && ai < bi); };
@@ -2036,6 +2059,23 @@ __device__ void sort_segment(uint32_t* __restrict__ it, O* __restrict__ d, uint6
     it[0] = it[e];
     it[e] = ti;
     sift(0, e);
+  }
+}
+
+// sort_segment with 5 to 16 entries sorted in registers (one read and one write of each
+// entry; insertion sort in global memory moves entries many times: on 250M results in
+// segments of 15.6 on average it wrote 39 GB of HBM for 3 GB of results). Takes ~130 VGPRs
+// for f64, so it runs in a kernel of its own (k_sort_segments).
+template <class O>
+__device__ void sort_segment_any(uint32_t* __restrict__ it, O* __restrict__ d, uint64_t len) {
+  if (len > 4 && len <= 8) {
+    sort_segment_regs<O, 8>(it, d, uint32_t(len));
+  } else if (len > 8 && len <= 16) {
+    sort_segment_regs<O, 16>(it, d, uint32_t(len));
+  } else if (len > 16 && len <= 32) {
+    sort_segment_regs<O, 32>(it, d, uint32_t(len));
+  } else {
+    sort_segment(it, d, len);
   }
 }
 

@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_within_count(DevTree<A, K> t, const 
 template <class A, int K, class M>
 __global__ __launch_bounds__(kBlock) void k_within_fill(DevTree<A, K> t, const A* __restrict__ queries,
                                                         const uint32_t* __restrict__ order, uint64_t m, M met,
-                                                        typename Ax<A>::O radius, int excl, int sorted,
+                                                        typename Ax<A>::O radius, int excl,
                                                         const uint64_t* __restrict__ offsets,
                                                         uint32_t* __restrict__ out_items,
                                                         typename Ax<A>::O* __restrict__ out_d,
@@ -269,9 +269,21 @@ __global__ __launch_bounds__(kBlock) void k_within_fill(DevTree<A, K> t, const A
     const uint64_t o = offsets[qi], cap = offsets[qi + 1] - o;
     WithinVis<O> v{radius, excl != 0, 0, out_items + o, out_d + o, cap};
     traverse(t, q, mq, v, cnt);
-    if (sorted) sort_segment(out_items + o, out_d + o, v.count < cap ? v.count : cap);
   }
   flush_counters(cnt, stats);
+}
+
+// Sorts each query's segment of a within result by (distance, item), one thread per query
+// (sort_segment_any: short segments in registers). A pass of its own after the fill, so the
+// fill kernels keep their register budget (with the sort network inlined the f64 fill took
+// 130 VGPRs, 3 waves per SIMD, instead of 40).
+template <class O>
+__global__ __launch_bounds__(kBlock) void k_sort_segments(const uint64_t* __restrict__ offsets, uint64_t m,
+                                                          uint32_t* __restrict__ items, O* __restrict__ d) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t o = offsets[i];
+  sort_segment_any(items + o, d + o, offsets[i + 1] - o);
 }
 
 // within count, G lanes per query (SquaredEuclidean): the group form of k_within_count, as
@@ -309,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_within_count_group(DevTree<A, K> t, 
 template <class A, int K, int G>
 __global__ __launch_bounds__(kBlock) void k_within_fill_group(DevTree<A, K> t, const A* __restrict__ queries,
                                                               const uint32_t* __restrict__ order, uint64_t m,
-                                                              typename Ax<A>::O radius, int excl, int sorted,
+                                                              typename Ax<A>::O radius, int excl,
                                                               const uint64_t* __restrict__ offsets,
                                                               uint32_t* __restrict__ out_items,
                                                               typename Ax<A>::O* __restrict__ out_d) {
@@ -331,7 +343,6 @@ __global__ __launch_bounds__(kBlock) void k_within_fill_group(DevTree<A, K> t, c
   Counters c;
   traverse(t, q, qw, v, c);
   v.flush(true);
-  if (sorted && gl == 0) sort_segment(out_items + o, out_d + o, v.count < cap ? v.count : cap);
 }
 
 }  // namespace dev

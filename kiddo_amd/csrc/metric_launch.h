@@ -209,7 +209,7 @@ template <class A, int K, class M>
 __global__ __launch_bounds__(kBlock) void k_periodic_within_fill(DevTree<A, K> t, const A* __restrict__ queries,
                                                                  const uint32_t* __restrict__ order, uint64_t m,
                                                                  M met, PBox<A> box, typename Ax<A>::O radius,
-                                                                 int excl, int sorted,
+                                                                 int excl,
                                                                  const uint64_t* __restrict__ offsets,
                                                                  uint32_t* __restrict__ out_items,
                                                                  typename Ax<A>::O* __restrict__ out_d,
@@ -219,7 +219,6 @@ __global__ __launch_bounds__(kBlock) void k_periodic_within_fill(DevTree<A, K> t
     const uint64_t o = offsets[qi], cap = offsets[qi + 1] - o;
     WithinVis<O> v{radius, excl != 0, 0, out_items + o, out_d + o, cap};
     periodic_traverse(t, q, pq, v, cnt);
-    if (sorted) sort_segment(out_items + o, out_d + o, v.count < cap ? v.count : cap);
   }
   flush_counters(cnt, stats);
 }
@@ -429,32 +428,36 @@ int mlaunch_within_fill(const QueryArgs& a, M met, double radius, const uint64_t
     using O = typename dev::Ax<A>::O;
     auto dt = dev_tree<A, KK>(a.t);
     KernelTimer kt(a.t, a.s);
+    bool launched = false;
     if constexpr (std::is_same<M, dev::SqEuclidean>::value) {
       // G lanes per query with coalesced result writes: by default for dense outputs (16
       // lanes), or as tuned (kd_tree_tune_within)
       const int g = a.t->fill_group ? a.t->fill_group : (a.dense ? 16 : 1);
-      auto group = [&](auto gc) -> int {
+      auto group = [&](auto gc) {
         constexpr int G = decltype(gc)::value;
         hipLaunchKernelGGL((dev::k_within_fill_group<A, KK, G>), dim3(grid_for(a.m * G)), dim3(dev::kBlock),
                            lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, O(radius),
-                           (a.flags & KD_EXCLUSIVE) ? 1 : 0, (a.flags & KD_UNSORTED) ? 0 : 1, offsets, items,
-                           static_cast<O*>(dists));
-        KD_HIP(hipGetLastError());
-        return KD_OK;
+                           (a.flags & KD_EXCLUSIVE) ? 1 : 0, offsets, items, static_cast<O*>(dists));
+        launched = true;
       };
       switch (g) {
-        case 4: return group(std::integral_constant<int, 4>{});
-        case 8: return group(std::integral_constant<int, 8>{});
-        case 16: return group(std::integral_constant<int, 16>{});
-        case 32: return group(std::integral_constant<int, 32>{});
+        case 4: group(std::integral_constant<int, 4>{}); break;
+        case 8: group(std::integral_constant<int, 8>{}); break;
+        case 16: group(std::integral_constant<int, 16>{}); break;
+        case 32: group(std::integral_constant<int, 32>{}); break;
         default: break;  // 1: one thread per query
       }
     }
-    hipLaunchKernelGGL((dev::k_within_fill<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds_bytes_for(dt),
-                       a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, O(radius),
-                       (a.flags & KD_EXCLUSIVE) ? 1 : 0, (a.flags & KD_UNSORTED) ? 0 : 1, offsets, items,
-                       static_cast<O*>(dists), nullptr);
+    if (!launched)
+      hipLaunchKernelGGL((dev::k_within_fill<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds_bytes_for(dt),
+                         a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met, O(radius),
+                         (a.flags & KD_EXCLUSIVE) ? 1 : 0, offsets, items, static_cast<O*>(dists), nullptr);
     KD_HIP(hipGetLastError());
+    if (!(a.flags & KD_UNSORTED) && a.m) {
+      hipLaunchKernelGGL((dev::k_sort_segments<O>), dim3(grid_for(a.m)), dim3(dev::kBlock), 0, a.s, offsets, a.m,
+                         items, static_cast<O*>(dists));
+      KD_HIP(hipGetLastError());
+    }
   });
   return KD_OK;
 }
@@ -538,9 +541,14 @@ int mlaunch_periodic_within_fill(const QueryArgs& a, M met, const void* box, dou
     KernelTimer kt(a.t, a.s);
     hipLaunchKernelGGL((dev::k_periodic_within_fill<A, KK, M>), dim3(grid_for(a.m)), dim3(dev::kBlock),
                        lds_bytes_for(dt), a.s, dt, static_cast<const A*>(a.queries), a.order, a.m, met,
-                       pbox_of<A, KK>(box), O(radius), (a.flags & KD_EXCLUSIVE) ? 1 : 0,
-                       (a.flags & KD_UNSORTED) ? 0 : 1, offsets, items, static_cast<O*>(dists), nullptr);
+                       pbox_of<A, KK>(box), O(radius), (a.flags & KD_EXCLUSIVE) ? 1 : 0, offsets, items,
+                       static_cast<O*>(dists), nullptr);
     KD_HIP(hipGetLastError());
+    if (!(a.flags & KD_UNSORTED) && a.m) {
+      hipLaunchKernelGGL((dev::k_sort_segments<O>), dim3(grid_for(a.m)), dim3(dev::kBlock), 0, a.s, offsets, a.m,
+                         items, static_cast<O*>(dists));
+      KD_HIP(hipGetLastError());
+    }
   });
   return KD_OK;
 }
