@@ -2062,10 +2062,11 @@ __device__ void sort_segment(uint32_t* __restrict__ it, O* __restrict__ d, uint6
   }
 }
 
-// sort_segment with 5 to 16 entries sorted in registers (one read and one write of each
+// sort_segment with 5 to 32 entries sorted in registers (one read and one write of each
 // entry; insertion sort in global memory moves entries many times: on 250M results in
-// segments of 15.6 on average it wrote 39 GB of HBM for 3 GB of results). Takes ~130 VGPRs
-// for f64, so it runs in a kernel of its own (k_sort_segments).
+// segments of 15.6 on average the sort wrote 17.2 GB of HBM for 3 GB of results, 3.8 GB with
+// 5 to 32 entries in registers). Takes 141 VGPRs for f64, so it runs in a kernel of its own
+// (k_sort_segments).
 template <class O>
 __device__ void sort_segment_any(uint32_t* __restrict__ it, O* __restrict__ d, uint64_t len) {
   if (len > 4 && len <= 8) {

@@ -275,8 +275,8 @@ __global__ __launch_bounds__(kBlock) void k_within_fill(DevTree<A, K> t, const A
 
 // Sorts each query's segment of a within result by (distance, item), one thread per query
 // (sort_segment_any: short segments in registers). A pass of its own after the fill, so the
-// fill kernels keep their register budget (with the sort network inlined the f64 fill took
-// 130 VGPRs, 3 waves per SIMD, instead of 40).
+// fill kernels keep their register budget (with a 16-slot network inlined the f64 fill took
+// 130 VGPRs, 3 waves per SIMD, instead of 76 and 6).
 template <class O>
 __global__ __launch_bounds__(kBlock) void k_sort_segments(const uint64_t* __restrict__ offsets, uint64_t m,
                                                           uint32_t* __restrict__ items, O* __restrict__ d) {
