@@ -1230,11 +1230,20 @@ __device__ __forceinline__ float round_up_f32(double x) {
 //     A list that is still full after compaction cannot prove it holds the whole answer:
 //     the query is marked for the exact redo pass (k_nearest_n_redo) and its traversal ends
 //     (every subtree pruned).
+//   * A listed entry's FP32 distance goes into top[] later: each visit inserts at most kIns
+//     of the list's pending entries (the first ones not yet inserted), and everything
+//     pending is inserted before a compaction and at the end. top[] holding a subset of the
+//     certified entries seen keeps the bound an upper bound on the answer's k-th distance; an
+//     entry that compaction drops fails a threshold at least the k-th distance of that
+//     subset, so inserting it would have changed nothing.
 //   * At the end the block's entries and the listed ones that pass the final threshold are
 //     measured in FP64 and the k smallest in (distance, item) order are kept (Batcher's
 //     network for the first MAXK, insertions for the rest): the plain kernel's answer.
 // A survivor of the screen costs a med3 per slot of top[] and one LDS store, instead of an
-// FP64 measurement and an insertion into an FP64 (distance, item) list.
+// FP64 measurement and an insertion into an FP64 (distance, item) list. Inserting each
+// survivor as it was found made a wave step through a visit's survivors at the pace of its
+// busiest lane, with the insertion in every step (87.7 wave steps for 10.6 lane steps per
+// query); appending is cheap, and the insertions are spread over the visits.
 template <int MAXK> struct Screen32Vis {
   static_assert(MAXK <= 16, "the FP32 collector is for k <= 16");
   // candidate list entries per lane: k + 16 (fewer compactions than k + 8: 15.4 against
@@ -1243,6 +1252,10 @@ template <int MAXK> struct Screen32Vis {
   static constexpr int kWords = MAXK >= 16 ? 2 : 1;
   static constexpr uint32_t kMaskN = 64u * kWords;
   static constexpr float kC = 1.0f + 0x1p-16f;
+  // pending list entries inserted into top[] per visit (measured on 16M queries at k = 16:
+  // 1 / 2 / 3 / 4 / 5 / 6 / 8: 21.9 / 18.5 / 15.50 / 15.24 / 15.18 / 15.25 / 15.43 ms, 15.67 ms
+  // with each survivor inserted as it was found)
+  static constexpr uint32_t kIns = 5;
   float top[MAXK];  // ascending; see init
   int k;
   double radius;
@@ -1257,6 +1270,7 @@ template <int MAXK> struct Screen32Vis {
   float thr;      // FP32 threshold of the current bound
   double bnd;     // min(U, radius); -inf after an overflow
   uint32_t nc;    // candidates in the list
+  uint32_t ins;   // list entries [0, ins) are in top[], [ins, nc) pending
   uint32_t visits, rescored;
   uint32_t* cand;  // LDS: kCand x kBlock entry indices, slot j of lane l at j * kBlock + l
 
@@ -1273,6 +1287,7 @@ template <int MAXK> struct Screen32Vis {
     pre = false;
     ovf = false;
     nc = 0;
+    ins = 0;
     visits = 0;
     rescored = 0;
     cand = cand_lds;
@@ -1323,7 +1338,7 @@ template <int MAXK> struct Screen32Vis {
   template <int K> __device__ __forceinline__ void compact(const DevTree<double, K>& t) {
     const uint32_t lane = threadIdx.x;
     constexpr uint32_t kChunk = 8;
-    uint32_t w = 0;
+    uint32_t w = 0, wi = 0;
     for (uint32_t j0 = 0; j0 < nc; j0 += kChunk) {
       uint32_t p[kChunk];
       float4 sh[kChunk];
@@ -1339,12 +1354,32 @@ template <int MAXK> struct Screen32Vis {
         // array, whenever the list was full, kept everything and kCand was not a multiple of 8.)
         if (keep) cand[w * kBlock + lane] = p[u];
         w += keep ? 1u : 0u;
+        wi += (keep && j0 + u < ins) ? 1u : 0u;
       }
     }
     nc = w;
+    ins = wi;
+  }
+  // inserts the next kIns pending list entries into top[] (their loads in flight together;
+  // positions past the list read entry 0 of the tree and insert nothing)
+  template <int K> __device__ __forceinline__ void insert_pending(const DevTree<double, K>& t) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t p[kIns];
+    float4 sh[kIns];
+#pragma unroll
+    for (uint32_t u = 0; u < kIns; ++u) p[u] = ins + u < nc ? cand[(ins + u) * kBlock + lane] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kIns; ++u) sh[u] = t.shadow[p[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < kIns; ++u) {
+      const float sc = shadow_sq<K>(sh[u], qf);
+      insert32((ins + u < nc) & (sc <= rlow) ? sc : __builtin_huge_valf());
+    }
+    ins = ins + kIns < nc ? ins + kIns : nc;
   }
   template <int K> __device__ __forceinline__ void append(const DevTree<double, K>& t, uint32_t p) {
     if (nc == kCand) {
+      while (ins < nc) insert_pending(t);
       refresh();
       compact(t);
       if (nc == kCand) {
@@ -1383,10 +1418,10 @@ template <int MAXK> struct Screen32Vis {
     }
   }
   // A visit after the pre-scan block: the screen marks the survivors among the first kMaskN
-  // entries against the threshold of the start of the visit, entries past the mask are
-  // screened one at a time; each survivor's FP32 distance goes into top[] and its index into
-  // the candidate list. The bound and threshold are refreshed at the end of the visit (and
-  // after each entry past the mask, for long leaves of equal coordinates).
+  // entries against the threshold of the start of the visit and they are appended to the
+  // candidate list, then up to kIns pending list entries go into top[]; entries past the mask
+  // are screened one at a time. The bound and threshold are refreshed at the end of the visit
+  // (and after each entry past the mask, for long leaves of equal coordinates).
   template <int K, class QW>
   __device__ __forceinline__ void leaf(const DevTree<double, K>& t, const QW&, uint32_t leaf, uint32_t nl) {
     const bool block = visits == 0u && pre;
@@ -1400,33 +1435,15 @@ template <int MAXK> struct Screen32Vis {
     if (!screen) mk.set_first(nm);
     screen_into(t, s, screen ? nm : 0u, mk);
     uint32_t next = nm;  // first entry not covered by the mask
-    uint32_t p = 0;
-    bool have = true;
-    auto select = [&]() {
-      have = true;
-      if (mk.any()) {
-        p = s + mk.pop();
-      } else {
-        next = next_tail(t, s, next, L);
-        if (next < L) {
-          p = s + next;
-          ++next;
-        } else {
-          have = false;
-        }
-      }
-    };
-    select();
-    float sn = have ? screen_dist(t, p) : 0.0f;
-    while (have) {
-      const uint32_t pc = p;
-      const float sc = sn;
-      select();
-      sn = screen_dist(t, have ? p : pc);  // the next survivor's load, a step ahead
-      insert32(sc <= rlow ? sc : __builtin_huge_valf());
-      append(t, pc);
-      have = have && !ovf;
-      if (pc - s + 1u >= kMaskN) refresh();
+    while (mk.any() && !ovf) append(t, s + mk.pop());
+    if (!ovf) insert_pending(t);
+    while (!ovf) {
+      next = next_tail(t, s, next, L);
+      if (next >= L) break;
+      append(t, s + next);
+      ++next;
+      if (!ovf) insert_pending(t);
+      refresh();
     }
     refresh();
   }
@@ -1436,6 +1453,8 @@ template <int MAXK> struct Screen32Vis {
   template <int K, class QW>
   __device__ __forceinline__ void finish(const DevTree<double, K>& t, const QW& qw, uint32_t bb, uint32_t be,
                                          TopK<double, MAXK>& res, int& cnt) {
+    while (ins < nc) insert_pending(t);
+    refresh();
     compact(t);
     const uint32_t lane = threadIdx.x;
     const uint32_t L = be - bb, nm = L < kMaskN ? L : kMaskN;
