@@ -717,22 +717,38 @@ def _traffic(name):
 def bench_mixture(args, torch, ka):
     """ImmutableKdTree<f32, 3>: 10,000,000 points from a 64-component isotropic Gaussian
     mixture in the unit cube (centres uniform, sigma 0.01, clipped), bucket 32; 10,000,000
-    queries from the same mixture; nearest_n(16), and within(r = 0.005, passed as r^2 =
-    2.5e-5) unsorted, whose ~1.8e10 results are produced in query chunks that fit
+    queries from the same mixture; nearest_n(16) with the FP32 collector (KD_SCREEN_F32 on
+    the tree's FP32 shadow), next to the plain f32 kernel in the same Morton order (both
+    autotuned, results checked equal); and within(r = 0.005, passed as r^2 = 2.5e-5)
+    unsorted, whose ~1.8e10 results are produced in query chunks that fit
     `--result-budget` bytes (count pass, then one fill pass per chunk)."""
     from kiddo_amd.datasets import gaussian_mixture
     n = m = 10_000_000
     centres = np.random.default_rng(args.seed ^ 0x5EED).random((64, 3))
     pts = gaussian_mixture(n, 64, 0.01, 3, seed=args.seed, centres=centres)
     qh = gaussian_mixture(m, 64, 0.01, 3, seed=args.seed + 1, centres=centres)
-    tree = ka.ImmutableKdTree.new_from_slice(pts, args.bucket)
+    tree = ka.ImmutableKdTree.new_from_slice(pts, args.bucket, shadow_f32=True)
     tree.set_timing(True)
     q = torch.from_numpy(qh).cuda()
-    flags = ka.KD_MORTON
+    # the plain f32 kernel, autotuned, in Morton order
+    chosen_plain = tree.autotune_nearest_n(q, 16, math.inf, ka.KD_MORTON)
+    dtp, kmsp, outp = _time_knn(tree, q, 16, ka.KD_MORTON, args.steps, args.warmup, torch)
+    stp = tree.query_stats()
+    log(f"mixture knn16 plain: {dtp * 1e3:.1f} ms/step ({m / dtp / 1e6:.1f} Mq/s), kernel {kmsp:.1f} ms")
+    # the FP32 collector, autotuned
+    flags = ka.KD_MORTON | ka.KD_SCREEN_F32
     chosen = tree.autotune_nearest_n(q, 16, math.inf, flags)
-    dt, kms, _ = _time_knn(tree, q, 16, flags, args.steps, args.warmup, torch)
+    dt, kms, outc = _time_knn(tree, q, 16, flags, args.steps, args.warmup, torch)
     st = tree.query_stats()
-    log(f"mixture knn16: {dt * 1e3:.1f} ms/step ({m / dt / 1e6:.1f} Mq/s), kernel {kms:.1f} ms, {st}")
+    same = bool(torch.equal(outp.items, outc.items) and torch.equal(outp.distances, outc.distances)
+                and torch.equal(outp.counts, outc.counts))
+    redos = tree.query_redos()
+    del outp, outc
+    log(f"mixture knn16 collector: {dt * 1e3:.1f} ms/step ({m / dt / 1e6:.1f} Mq/s), kernel {kms:.1f} ms, "
+        f"equal to plain {same}, redone {redos}, {st}")
+    if not same:
+        log("collector results differ from the plain kernel's")
+        sys.exit(3)
     # within, chunked by result memory; launch shapes of both passes autotuned on a sample
     r2 = 0.005 ** 2
     wflags = ka.KD_UNSORTED | ka.KD_MORTON
@@ -801,7 +817,9 @@ def bench_mixture(args, torch, ka):
         "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded mixture)",
         "config": {"workload": "mixture-10M-f32-knn16+within0.005", "points": n, "queries": m,
-                   "k": 16, "launch": chosen},
+                   "k": 16, "launch": chosen, "variant": "FP32 collector, Morton-ordered"},
+        "plain_f32_qps": round(m / dtp, 1), "plain_f32_kernel_ms": round(kmsp, 3), "plain_f32_launch": chosen_plain,
+        "collector_over_plain": round(dtp / dt, 3), "equal_to_plain": same, "redone": redos,
         "within_r0.005_unsorted": {"results": total, "chunks": chunks,
                                    "result_budget_bytes": args.result_budget,
                                    "launch": wshape,
@@ -828,8 +846,8 @@ def bench_mixture(args, torch, ka):
         "roofline": {"bound": "hbm", "achieved": round(algo / (kms / 1e3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(algo / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": tr.get("k_nearest_n<float,3,16>", {}).get("bytes_per_launch"),
-                     "kernel": "k_nearest_n<float,3,16>", "kernel_ms": round(kms, 3)},
+                     "traffic": tr.get("k_nearest_n_screen32<float,3,16>", {}).get("bytes_per_launch"),
+                     "kernel": "k_nearest_n_screen32<float,3,16>", "kernel_ms": round(kms, 3)},
     }), flush=True)
 
 

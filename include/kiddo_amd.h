@@ -49,7 +49,8 @@ enum kd_query_flags {
   KD_EXCLUSIVE = 1u << 0, /* `.exclusive_boundaries()`: strict `< radius`              */
   KD_UNSORTED = 1u << 1,  /* `.unsorted()`: within results in visit order              */
   KD_MORTON = 1u << 2,    /* traverse queries in Morton order (results keep input order) */
-  KD_SCREEN_F32 = 1u << 3, /* f64 trees: FP32 screen + FP64 rescore (same results)     */
+  KD_SCREEN_F32 = 1u << 3, /* k-NN with the FP32 collector (same results): f64 trees  */
+                           /* screened in FP32, rescored in FP64; f32 trees exactly     */
   KD_GRAPH = 1u << 4        /* forests: capture the launch in a hipGraph and replay it     */
 };
 
@@ -71,7 +72,7 @@ enum kd_query_flags {
 
 /* tree-create flags */
 enum kd_tree_flags {
-  KD_TREE_SHADOW_F32 = 1u << 0 /* f64 trees: keep an FP32 shadow of the leaves            */
+  KD_TREE_SHADOW_F32 = 1u << 0 /* f64 and f32 trees: keep an FP32 shadow of the leaves    */
 };
 
 /* A tree in Kiddo's ImmutableKdTree layout, exactly what a Rust shim reads through the
@@ -179,8 +180,9 @@ int kd_tree_tune_within(kd_tree* tree, int32_t count_group, int32_t fill_group);
 int kd_tree_tune_best_n_within(kd_tree* tree, int32_t group);
 /* Autotuner of the within passes: times each shape of the count pass on up to 2^20 of the
  * queries (with KD_MORTON: 16 runs of their Morton order), then each shape of the fill pass
- * on as many of them as 2^28 results allow, and keeps the fastest of each in the tree
- * (chosen2 = {count shape, fill shape}). Synchronous. */
+ * on as many of them as 2^29 results allow, and keeps the fastest of each in the tree (a
+ * fill shape with more lanes per query only when more than 10% faster than the best with
+ * fewer) (chosen2 = {count shape, fill shape}). Synchronous. */
 int kd_autotune_within(kd_tree* tree, const void* queries, uint64_t m, double radius, uint32_t flags,
                        int32_t* chosen2, void* stream);
 
@@ -219,8 +221,10 @@ int kd_approx_nearest_one(kd_tree* tree, const void* queries, uint64_t m, uint32
  * (builder.rs:2019-2042; query/nearest_n.rs:21-38, nearest_n_within.rs:127-179).
  * radius = +INFINITY for plain nearest_n. Row-major m x k outputs; row i holds counts[i]
  * results sorted by (distance, item) and is padded with item 0xFFFFFFFF, distance +inf.
- * KD_SCREEN_F32 (f64 trees created with KD_TREE_SHADOW_F32) screens in FP32 against a
- * certified error bound and rescores survivors in FP64: results are identical. */
+ * KD_SCREEN_F32 (trees created with KD_TREE_SHADOW_F32): on f64 trees the traversal screens
+ * in FP32 against a certified error bound and rescores survivors in FP64; on f32 trees
+ * (k <= 16) it keeps FP32 distances only and resolves items at the end. Results are
+ * identical to the call without the flag. */
 int kd_nearest_n(kd_tree* tree, const void* queries, uint64_t m, uint32_t k, double radius,
                  uint32_t flags, uint32_t* items, void* dists, uint32_t* counts, void* stream);
 

@@ -770,6 +770,15 @@ __device__ __forceinline__ void screen_setup(const Small<double, K>& q, const do
   eps_k = sqrt((double)K) * eps * (1.0 + 1e-6);
 }
 
+// f32 trees: the query itself, and no error term (the shadow holds the coordinates exactly)
+template <int K>
+__device__ __forceinline__ void screen_setup(const Small<float, K>& q, const double* __restrict__,
+                                             float (&qf)[4], double& eps_k) {
+#pragma unroll
+  for (int d = 0; d < 4; ++d) qf[d] = d < K ? q[d] : 0.0f;
+  eps_k = 0.0;
+}
+
 // ------------------------------------------------ collectors for large k or n --------
 // Above the register forms (k or n > 64) the collector is a binary max-heap held in the
 // query's own output row (k entries of HBM, cached in L2), as Kiddo's BinaryHeap collector
@@ -1244,8 +1253,13 @@ __device__ __forceinline__ float round_up_f32(double x) {
 // survivor as it was found made a wave step through a visit's survivors at the pace of its
 // busiest lane, with the insertion in every step (87.7 wave steps for 10.6 lane steps per
 // query); appending is cheap, and the insertions are spread over the visits.
-template <int MAXK> struct Screen32Vis {
+template <int MAXK, class A = double> struct Screen32Vis {
   static_assert(MAXK <= 16, "the FP32 collector is for k <= 16");
+  using O = typename Ax<A>::O;
+  // f32 trees: the shadow distance is the tree's own FP32 distance (same operations in the
+  // same order), so nothing needs certifying: the bound and the threshold are the k-th
+  // distance of top[] itself, and radius, bound and distances are FP32
+  static constexpr bool kExact = std::is_same<A, float>::value;
   // candidate list entries per lane: k + 16 (fewer compactions than k + 8: 15.4 against
   // 16.0 ms on 16M queries at k = 16, with the LDS stems cut to 8 KiB to keep 4 blocks per CU)
   static constexpr uint32_t kCand = MAXK == 1 ? 8u : uint32_t(MAXK) + 16u;
@@ -1258,7 +1272,7 @@ template <int MAXK> struct Screen32Vis {
   static constexpr uint32_t kIns = 5;
   float top[MAXK];  // ascending; see init
   int k;
-  double radius;
+  O radius;
   bool excl;
   bool pre;       // the first visit is the pre-scan block, whose distances are in top already
   bool ovf;       // the candidate list overflowed: the query goes to the redo pass
@@ -1268,14 +1282,14 @@ template <int MAXK> struct Screen32Vis {
   float rlow;     // certify_limit(radius)
   float thr_r;    // screen_threshold(radius)
   float thr;      // FP32 threshold of the current bound
-  double bnd;     // min(U, radius); -inf after an overflow
+  O bnd;          // min(U, radius); -inf after an overflow
   uint32_t nc;    // candidates in the list
   uint32_t ins;   // list entries [0, ins) are in top[], [ins, nc) pending
   uint32_t visits, rescored;
   uint32_t* cand;  // LDS: kCand x kBlock entry indices, slot j of lane l at j * kBlock + l
 
   // after qf and eps_k are set
-  __device__ void init(int k_, double r, bool ex, uint32_t* cand_lds) {
+  __device__ void init(int k_, O r, bool ex, uint32_t* cand_lds) {
     // the first MAXK - k slots hold -inf and never leave the front, so the last slot is
     // always the k-th smallest distance (a constant index: a run-time one puts top[] in
     // scratch memory)
@@ -1291,20 +1305,31 @@ template <int MAXK> struct Screen32Vis {
     visits = 0;
     rescored = 0;
     cand = cand_lds;
-    e1 = round_up_f32(eps_k * (1.0 + 0x1p-16));
-    e2 = round_up_f32(2.0 * eps_k * (1.0 + 0x1p-16));
-    rlow = certify_limit(r, eps_k);
-    thr_r = screen_threshold(r, eps_k);
+    if constexpr (kExact) {
+      e1 = e2 = 0.0f;
+      rlow = r;
+      thr_r = r;
+    } else {
+      e1 = round_up_f32(eps_k * (1.0 + 0x1p-16));
+      e2 = round_up_f32(2.0 * eps_k * (1.0 + 0x1p-16));
+      rlow = certify_limit(r, eps_k);
+      thr_r = screen_threshold(r, eps_k);
+    }
     bnd = r;
     thr = thr_r;
   }
-  __device__ double bound() const { return bnd; }
-  __device__ bool prune(double rd) const { return prune_at(rd, bnd); }
-  __device__ bool prune_at(double rd, double b) const { return ovf || rd > b || (excl && !(rd < radius)); }
-  __device__ bool unbounded() const { return bnd == __builtin_huge_val(); }
+  __device__ O bound() const { return bnd; }
+  __device__ bool prune(O rd) const { return prune_at(rd, bnd); }
+  __device__ bool prune_at(O rd, O b) const { return ovf || rd > b || (excl && !(rd < radius)); }
+  __device__ bool unbounded() const { return bnd == oinf<O>(); }
   // bound and threshold from top[] (called when top[] may have changed)
   __device__ __forceinline__ void refresh() {
     const float L = top[MAXK - 1];
+    if constexpr (kExact) {
+      bnd = ovf ? -oinf<O>() : (L < radius ? L : radius);
+      thr = L < thr_r ? L : thr_r;
+      return;
+    }
     float U32 = __builtin_huge_valf(), t32 = __builtin_huge_valf();
     if (L < __builtin_huge_valf()) {
       const float s = sqrtf(L + 0x1p-146f) * kC;
@@ -1312,8 +1337,8 @@ template <int MAXK> struct Screen32Vis {
       U32 = u * u * kC;
       t32 = t * t * kC + 0x1p-146f;
     }
-    const double U = double(U32);
-    bnd = ovf ? -__builtin_huge_val() : (U < radius ? U : radius);
+    const O U = O(U32);
+    bnd = ovf ? -oinf<O>() : (U < radius ? U : radius);
     thr = t32 < thr_r ? t32 : thr_r;
   }
   // sorted insertion of a distance (+inf changes nothing)
@@ -1323,11 +1348,11 @@ template <int MAXK> struct Screen32Vis {
     top[0] = fminf(top[0], c);
   }
   template <int K>
-  __device__ __forceinline__ float screen_dist(const DevTree<double, K>& t, uint32_t p) const {
+  __device__ __forceinline__ float screen_dist(const DevTree<A, K>& t, uint32_t p) const {
     return shadow_sq<K>(t.shadow[p], qf);
   }
   template <int K>
-  __device__ __forceinline__ uint32_t next_tail(const DevTree<double, K>& t, uint32_t s, uint32_t next,
+  __device__ __forceinline__ uint32_t next_tail(const DevTree<A, K>& t, uint32_t s, uint32_t next,
                                                 uint32_t L) const {
     while (next < L && screen_dist(t, s + next) > thr) ++next;
     return next;
@@ -1335,7 +1360,7 @@ template <int MAXK> struct Screen32Vis {
   // keeps the candidates that pass the current threshold, in order; 8 at a time, with their
   // shadow loads in flight together (one at a time, a compaction was a chain of 24 dependent
   // loads, which the wave of a lane whose list filled waited for)
-  template <int K> __device__ __forceinline__ void compact(const DevTree<double, K>& t) {
+  template <int K> __device__ __forceinline__ void compact(const DevTree<A, K>& t) {
     const uint32_t lane = threadIdx.x;
     constexpr uint32_t kChunk = 8;
     uint32_t w = 0, wi = 0;
@@ -1362,7 +1387,7 @@ template <int MAXK> struct Screen32Vis {
   }
   // inserts the next kIns pending list entries into top[] (their loads in flight together;
   // positions past the list read entry 0 of the tree and insert nothing)
-  template <int K> __device__ __forceinline__ void insert_pending(const DevTree<double, K>& t) {
+  template <int K> __device__ __forceinline__ void insert_pending(const DevTree<A, K>& t) {
     const uint32_t lane = threadIdx.x;
     uint32_t p[kIns];
     float4 sh[kIns];
@@ -1377,14 +1402,14 @@ template <int MAXK> struct Screen32Vis {
     }
     ins = ins + kIns < nc ? ins + kIns : nc;
   }
-  template <int K> __device__ __forceinline__ void append(const DevTree<double, K>& t, uint32_t p) {
+  template <int K> __device__ __forceinline__ void append(const DevTree<A, K>& t, uint32_t p) {
     if (nc == kCand) {
       while (ins < nc) insert_pending(t);
       refresh();
       compact(t);
       if (nc == kCand) {
         ovf = true;
-        bnd = -__builtin_huge_val();
+        bnd = -oinf<O>();
         return;
       }
     }
@@ -1394,7 +1419,7 @@ template <int MAXK> struct Screen32Vis {
   // marks in mk the entries among [s, s + nm) whose FP32 distance passes thr (shadow loads
   // issued 4 at a time from one address, up to 3 past the range: the shadow's padding)
   template <int K>
-  __device__ __forceinline__ void screen_into(const DevTree<double, K>& t, uint32_t s, uint32_t nm,
+  __device__ __forceinline__ void screen_into(const DevTree<A, K>& t, uint32_t s, uint32_t nm,
                                               EntryMask<kWords>& mk) const {
     constexpr uint32_t kBatch = 4;
 #pragma unroll
@@ -1423,7 +1448,7 @@ template <int MAXK> struct Screen32Vis {
   // are screened one at a time. The bound and threshold are refreshed at the end of the visit
   // (and after each entry past the mask, for long leaves of equal coordinates).
   template <int K, class QW>
-  __device__ __forceinline__ void leaf(const DevTree<double, K>& t, const QW&, uint32_t leaf, uint32_t nl) {
+  __device__ __forceinline__ void leaf(const DevTree<A, K>& t, const QW&, uint32_t leaf, uint32_t nl) {
     const bool block = visits == 0u && pre;
     visits += 1u;
     if (nl == 0u || ovf || block) return;
@@ -1451,8 +1476,8 @@ template <int MAXK> struct Screen32Vis {
   // the final threshold, measured in FP64, in (distance, item) order; ok entries are those
   // within the radius.
   template <int K, class QW>
-  __device__ __forceinline__ void finish(const DevTree<double, K>& t, const QW& qw, uint32_t bb, uint32_t be,
-                                         TopK<double, MAXK>& res, int& cnt) {
+  __device__ __forceinline__ void finish(const DevTree<A, K>& t, const QW& qw, uint32_t bb, uint32_t be,
+                                         TopK<O, MAXK>& res, int& cnt) {
     while (ins < nc) insert_pending(t);
     refresh();
     compact(t);
@@ -1495,14 +1520,14 @@ template <int MAXK> struct Screen32Vis {
     int nok = 0;
 #pragma unroll
     for (int j = 0; j < MAXK; ++j) {
-      const double dd = point_dist(t, pidx[j], qw);
+      const O dd = point_dist(t, pidx[j], qw);
       const uint32_t it = t.items[pidx[j]];
       const bool ok = (uint32_t(j) < taken) & ((dd < radius) | (!excl & (dd == radius)));
       // the item is masked rather than selected (a select of a loaded value becomes a
       // branch around its load); the empty asm keeps the mask opaque
       uint32_t drop = ok ? 0u : 0xFFFFFFFFu;
       asm("" : "+v"(drop));
-      res.d[j] = ok ? dd : __builtin_huge_val();
+      res.d[j] = ok ? dd : oinf<O>();
       res.it[j] = it | drop;
       nok += ok ? 1 : 0;
     }
@@ -1510,11 +1535,11 @@ template <int MAXK> struct Screen32Vis {
     uint32_t p = bb;
     while (next_cand(p)) {
       ++taken;
-      const double dd = point_dist(t, p, qw);
+      const O dd = point_dist(t, p, qw);
       const uint32_t it = t.items[p];
       const bool ok = (dd < radius) | (!excl & (dd == radius));
       nok += ok ? 1 : 0;
-      res.insert_if(ok && TopK<double, MAXK>::less(dd, it, res.d[MAXK - 1], res.it[MAXK - 1]), dd, it);
+      res.insert_if(ok && TopK<O, MAXK>::less(dd, it, res.d[MAXK - 1], res.it[MAXK - 1]), dd, it);
     }
     rescored = taken;
     cnt = nok < k ? nok : k;

@@ -416,8 +416,8 @@ int build_device(kd_tree* t, int dtype, int dims, int bucket, uint64_t n, const 
   if (bucket < 1) return set_error(KD_ERR_INVALID, "bucket size must be >= 1");
   if (n > 0 && !points) return set_error(KD_ERR_INVALID, "points is null");
   if (n >= 0x7FFFFFFFull) return set_error(KD_ERR_UNSUPPORTED, "device builds hold < 2^31 entries");
-  if ((flags & KD_TREE_SHADOW_F32) && dtype != KD_F64)
-    return set_error(KD_ERR_UNSUPPORTED, "the FP32 shadow is provided for f64 trees");
+  if ((flags & KD_TREE_SHADOW_F32) && dtype != KD_F64 && dtype != KD_F32)
+    return set_error(KD_ERR_UNSUPPORTED, "the FP32 shadow is provided for f64 and f32 trees");
   t->dtype = dtype;
   t->dims = dims;
   t->bucket = bucket;

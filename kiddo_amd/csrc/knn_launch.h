@@ -198,26 +198,31 @@ int launch_thread_w(const QueryArgs& a, int k, double radius, uint32_t* items, v
   const size_t lds = size_t(dt.lds_n) * sizeof(A);
   reset_counters(t, a.s);
   KernelTimer kt(t, a.s);
+  constexpr bool kFloatTree = std::is_same<A, double>::value || std::is_same<A, float>::value;
+  if constexpr (kFloatTree) {
+  if ((a.flags & KD_SCREEN_F32) && MAXK <= 16) {
+    // the FP32 collector (f64: screened, f32: exact). LDS: the candidate list plus the stems
+    // within 40 KiB per block, so that 4 blocks (16 waves) fit a CU's 160 KiB: the stems take
+    // the largest power of two that fits
+    constexpr int MK = MAXK <= 16 ? MAXK : 16;
+    constexpr size_t kList = size_t(dev::Screen32Vis<MK, A>::kCand) * 4 * dev::kBlock;
+    auto st = dt;
+    while (st.lds_n && kList + size_t(st.lds_n) * sizeof(A) > 40960) st.lds_n >>= 1;
+    hipLaunchKernelGGL((dev::k_nearest_n_screen32<A, K, MK, WAVES>), dim3(grid_for(a.m)), dim3(dev::kBlock),
+                       size_t(st.lds_n) * sizeof(A), a.s, st, q, a.order, a.m, k, O(radius), excl, t->d_max_abs,
+                       items, static_cast<O*>(dists), counts, t->counters);
+    KD_HIP(hipGetLastError());
+    // the queries whose candidate list overflowed, answered exactly; global stems only
+    auto rt = dt;
+    rt.lds_n = 0;
+    const uint32_t rg = std::min<uint32_t>(grid_for(a.m), 1024u);
+    hipLaunchKernelGGL((dev::k_nearest_n_redo<A, K, MK>), dim3(rg), dim3(dev::kBlock), 0, a.s, rt, q, a.m, k,
+                       O(radius), excl, items, static_cast<O*>(dists), counts, t->counters);
+    KD_HIP(hipGetLastError());
+    return KD_OK;
+  }
+  }
   if constexpr (std::is_same<A, double>::value) {
-    if ((a.flags & KD_SCREEN_F32) && MAXK <= 16) {
-      // LDS: the candidate list plus the stems within 40 KiB per block, so that 4 blocks (16
-      // waves) fit a CU's 160 KiB: the stems take the largest power of two that fits
-      constexpr size_t kList = size_t(dev::Screen32Vis<(MAXK <= 16 ? MAXK : 16)>::kCand) * 4 * dev::kBlock;
-      auto st = dt;
-      while (st.lds_n && kList + size_t(st.lds_n) * sizeof(double) > 40960) st.lds_n >>= 1;
-      hipLaunchKernelGGL((dev::k_nearest_n_screen32<K, (MAXK <= 16 ? MAXK : 16), WAVES>), dim3(grid_for(a.m)),
-                         dim3(dev::kBlock), size_t(st.lds_n) * sizeof(double), a.s, st, q, a.order, a.m, k, radius,
-                         excl, t->d_max_abs, items, static_cast<double*>(dists), counts, t->counters);
-      KD_HIP(hipGetLastError());
-      // the queries whose candidate list overflowed, answered exactly; global stems only
-      auto rt = dt;
-      rt.lds_n = 0;
-      const uint32_t rg = std::min<uint32_t>(grid_for(a.m), 1024u);
-      hipLaunchKernelGGL((dev::k_nearest_n_redo<K, (MAXK <= 16 ? MAXK : 16)>), dim3(rg), dim3(dev::kBlock), 0, a.s,
-                         rt, q, a.m, k, radius, excl, items, static_cast<double*>(dists), counts, t->counters);
-      KD_HIP(hipGetLastError());
-      return KD_OK;
-    }
     if (a.flags & KD_SCREEN_F32) {
       hipLaunchKernelGGL((dev::k_nearest_n_screen<K, MAXK, WAVES>), dim3(grid_for(a.m)), dim3(dev::kBlock), lds, a.s,
                          dt, q, a.order, a.m, k, radius, excl, t->d_max_abs, items,
@@ -244,7 +249,8 @@ int launch_group(const QueryArgs& a, int k, double radius, uint32_t* items, void
   const uint32_t grid = grid_for(a.m * G);
   reset_counters(t, a.s);
   KernelTimer kt(t, a.s);
-  if (a.flags & KD_SCREEN_F32) {
+  // the group form screens f64 trees only; f32 trees take its plain kernel (the same results)
+  if ((a.flags & KD_SCREEN_F32) && std::is_same<A, double>::value) {
     if constexpr (std::is_same<A, double>::value) {
       hipLaunchKernelGGL((dev::k_knn_group<A, K, G, true>), dim3(grid), dim3(dev::kBlock), lds, a.s, dt, q,
                          a.order, a.m, k, O(radius), excl, t->d_max_abs, items, static_cast<O*>(dists), counts,
@@ -263,8 +269,8 @@ template <class A, int K>
 int launch_typed(const QueryArgs& a, int k, double radius, uint32_t* items, void* dists, uint32_t* counts) {
   kd_tree* t = a.t;
   if (a.flags & KD_SCREEN_F32) {
-    if (!(std::is_same<A, double>::value))
-      return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 applies to f64 trees");
+    if (!(std::is_same<A, double>::value || std::is_same<A, float>::value))
+      return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 applies to f64 and f32 trees");
     if (!t->shadow)
       return set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a tree created with KD_TREE_SHADOW_F32");
   }
