@@ -385,8 +385,10 @@ def main():
 def bench_forest(args, torch, ka):
     """Batched form: `--trees` independent f32 trees, sizes log-uniform in [1000, 262144]
     (seeded noisy unit-sphere clouds, radius jittered by N(0, 0.01)), each with a query set
-    of log-uniform size in [256, 65536] from the same law; nearest_n(k=8) and within(r=0.02,
-    passed as r^2 = 4e-4), one persistent launch per batch, replayed from a hipGraph."""
+    of log-uniform size in [256, 65536] from the same law; nearest_n(k=8) (the plain
+    collector; the FP32 collector, KD_SCREEN_F32 on the forest's FP32 shadow, timed beside
+    it and checked equal) and within(r=0.02, passed as r^2 = 4e-4), one persistent launch per
+    batch, replayed from a hipGraph."""
     from concurrent.futures import ThreadPoolExecutor
     from kiddo_amd.datasets import noisy_sphere
 
@@ -415,7 +417,7 @@ def bench_forest(args, torch, ka):
         lays = list(ex.map(lambda p: ka.HostLayout(p, bucket, None, 1), pts))
     log(f"host build {time.time() - t0:.1f}s")
     t0 = time.time()
-    forest = ka.Forest(lays)
+    forest = ka.Forest(lays, shadow_f32=True)
     del lays
     log(f"upload {time.time() - t0:.1f}s")
     offs = np.zeros(T + 1, np.uint64)
@@ -424,22 +426,34 @@ def bench_forest(args, torch, ka):
     q = torch.from_numpy(np.concatenate(qs)).cuda()
     forest.plan(q, offs, chunk=256, morton=True)
     k = 8
-    chosen = forest.autotune_nearest_n(q, k)
-    log(f"autotuned forest launch: {chosen}")
     forest.set_timing(True)
-    out = forest._knn_out(k)
-    for _ in range(args.warmup):
-        forest.nearest_n(q, k, flags=ka.KD_GRAPH, out=out)
-    torch.cuda.synchronize()
-    kms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        forest.nearest_n(q, k, flags=ka.KD_GRAPH, out=out)
-        kms.append(forest.last_kernel_ms())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    st = forest.query_stats()
-    log(f"forest knn: {1e3 * dt / args.steps:.1f} ms/step, {m * args.steps / dt / 1e6:.1f} Mq/s, {st}")
+
+    def timed_knn(flags):
+        chosen = forest.autotune_nearest_n(q, k, flags=flags)
+        out = forest._knn_out(k)
+        for _ in range(args.warmup):
+            forest.nearest_n(q, k, flags=flags | ka.KD_GRAPH, out=out)
+        torch.cuda.synchronize()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            forest.nearest_n(q, k, flags=flags | ka.KD_GRAPH, out=out)
+            kms.append(forest.last_kernel_ms())
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, kms, forest.query_stats(), chosen, out
+
+    dtc, kmsc, _, chosen_coll, outc = timed_knn(ka.KD_SCREEN_F32)
+    log(f"forest knn FP32 collector: {1e3 * dtc / args.steps:.1f} ms/step, {m * args.steps / dtc / 1e6:.1f} Mq/s, "
+        f"launch {chosen_coll}")
+    dt, kms, st, chosen, outp = timed_knn(0)
+    same = bool(torch.equal(outp.items, outc.items) and torch.equal(outp.distances, outc.distances)
+                and torch.equal(outp.counts, outc.counts))
+    del outp, outc
+    log(f"forest knn: {1e3 * dt / args.steps:.1f} ms/step, {m * args.steps / dt / 1e6:.1f} Mq/s, "
+        f"launch {chosen}, FP32 collector equal {same}, {st}")
+    if not same:
+        log("collector results differ from the plain kernel's")
+        sys.exit(3)
     # within(r = 0.02): count + scan + fill, timed together, best of 3
     wdt = float("inf")
     for _ in range(3):
@@ -483,6 +497,9 @@ def bench_forest(args, torch, ka):
         "config": {"workload": "forest-4096-f32-knn8", "trees": T, "points": int(sizes.sum()),
                    "queries": m, "k": k, "launch": chosen, "bucket": bucket,
                    "bucket_autotune_ms": bucket_ms},
+        "fp32_collector_qps": round(m * args.steps / dtc, 1),
+        "fp32_collector_kernel_ms": round(float(np.mean(kmsc)), 3), "fp32_collector_launch": chosen_coll,
+        "fp32_collector_equal": same,
         "within_r0.02": {"ms": round(wdt * 1e3, 2), "results": nres,
                          "results_per_s": round(nres / wdt, 1)},
         "roofline": {"bound": "hbm", "achieved": round(algo / kern_s / 1e9, 1), "peak": HBM_PEAK_GBS,

@@ -648,12 +648,14 @@ class Forest:
     """Many independent trees (same dtype and K) in pooled HBM buffers, queried as one batch
     by a persistent kernel fed from a device work queue (kd_forest_* in kiddo_amd.h)."""
 
-    def __init__(self, layouts):
-        """layouts: HostLayout objects (or anything with a `.desc` TreeDesc)."""
+    def __init__(self, layouts, shadow_f32: bool = False):
+        """layouts: HostLayout objects (or anything with a `.desc` TreeDesc). shadow_f32 (f32
+        trees): keep an FP32 shadow of the entries for the FP32 collector (KD_SCREEN_F32)."""
         self._layouts = list(layouts)
         arr = (TreeDesc * len(self._layouts))(*[l.desc for l in self._layouts])
         h = C.c_void_p()
-        _check(lib().kd_forest_create(arr, len(self._layouts), 0, C.byref(h)), "kd_forest_create")
+        _check(lib().kd_forest_create(arr, len(self._layouts), KD_TREE_SHADOW_F32 if shadow_f32 else 0,
+                                      C.byref(h)), "kd_forest_create")
         self._h = h
         self.ntrees = len(self._layouts)
         self.dtype_code = self._layouts[0].dtype_code
@@ -662,8 +664,8 @@ class Forest:
         self._layouts = None  # the device pools hold everything now
 
     @classmethod
-    def from_points(cls, point_sets, bucket=32, threads=0):
-        return cls([HostLayout(p, bucket, None, threads) for p in point_sets])
+    def from_points(cls, point_sets, bucket=32, threads=0, shadow_f32: bool = False):
+        return cls([HostLayout(p, bucket, None, threads) for p in point_sets], shadow_f32)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

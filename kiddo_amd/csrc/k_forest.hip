@@ -21,6 +21,7 @@
 #include <thread>
 #include <vector>
 
+#include "knn_screen.h"
 #include "tree.h"
 
 namespace kd {
@@ -47,6 +48,7 @@ template <class A, int K> struct ForestView {
   const uint32_t* leaf_start;
   const A* coords;
   const uint32_t* items;
+  const float4* shadow;  // pooled FP32 shadow (forests created with KD_TREE_SHADOW_F32) or null
   uint64_t plane;
   const ForestTree* trees;
 
@@ -57,7 +59,7 @@ template <class A, int K> struct ForestView {
     t.leaf_start = leaf_start + ft.leaf_off;
     t.coords = coords;
     t.items = items;
-    t.shadow = nullptr;
+    t.shadow = shadow;
     t.lds = nullptr;
     t.lds_n = 0;
     t.plane = plane;
@@ -104,6 +106,102 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     for (uint32_t j = threadIdx.x; j < wi.count; j += kBlock) {
       const uint64_t pos = uint64_t(wi.begin) + j;
       const uint64_t qi = order ? order[pos] : pos;
+      Small<A, K> q;
+      Small<O, K> qw;
+      load_query<A, K>(queries, qi, q, qw);
+      KnnVis<O, MAXK> v;
+      v.init(k, radius, excl != 0);
+      traverse(t, q, qw, v, cnt);
+      uint32_t* ri = out_items + qi * uint64_t(k);
+      O* rdd = out_d + qi * uint64_t(k);
+#pragma unroll
+      for (int jj = 0; jj < MAXK; ++jj)
+        if (jj < k) {
+          ri[jj] = jj < v.cnt ? v.top.it[jj] : 0xFFFFFFFFu;
+          rdd[jj] = jj < v.cnt ? v.top.d[jj] : oinf<O>();
+        }
+      out_counts[qi] = uint32_t(v.cnt);
+    }
+  }
+  flush_counters(cnt, stats);
+}
+
+// The FP32 collector (device.h Screen32Vis, exact on f32 trees) in the persistent form:
+// each lane answers a query of the item with FP32 distances only and resolves items at
+// the end. Queries whose candidate list overflowed get count kRedo and are answered by
+// k_forest_knn_redo, launched after this kernel on the same stream. The candidate list is
+// static LDS; the item's top stems take what is left of 40 KiB per block (lds_cap).
+template <class A, int K, int MAXK, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_forest_knn_s32(
+    ForestView<A, K> f, const WorkItem* __restrict__ work, uint32_t n_work, unsigned int* __restrict__ queue,
+    const A* __restrict__ queries, const uint32_t* __restrict__ order, int k, typename Ax<A>::O radius,
+    int excl, uint32_t lds_cap, uint32_t* __restrict__ out_items, typename Ax<A>::O* __restrict__ out_d,
+    uint32_t* __restrict__ out_counts, unsigned long long* __restrict__ stats) {
+  using O = typename Ax<A>::O;
+  __shared__ uint32_t cand[Screen32Vis<MAXK, A>::kCand * kBlock];
+  Counters cnt;
+  uint32_t w;
+  while (next_item(queue, n_work, w)) {
+    const WorkItem wi = work[w];
+    DevTree<A, K> t = f.tree(wi.tree);
+    if (lds_cap) pin_forest_stems(t, f.trees[wi.tree].stems_len, lds_cap);
+    for (uint32_t j = threadIdx.x; j < wi.count; j += kBlock) {
+      const uint64_t pos = uint64_t(wi.begin) + j;
+      const uint64_t qi = order ? order[pos] : pos;
+      Small<A, K> q;
+      Small<O, K> qw;
+      load_query<A, K>(queries, qi, q, qw);
+      Screen32Vis<MAXK, A> v;
+      screen_setup<K>(q, nullptr, v.qf, v.eps_k);
+      v.init(k, radius, excl != 0, cand);
+      int up = 0;
+      uint32_t bb = 0, be = 0;
+      if (t.n != 0) {
+        prescan_top<A, K, MAXK>(t, q, v, up, bb, be);
+        v.pre = true;
+      }
+      v.refresh();
+      traverse(t, q, qw, v, cnt, up);
+      uint32_t* ri = out_items + qi * uint64_t(k);
+      O* rdd = out_d + qi * uint64_t(k);
+      if (v.ovf) {
+        out_counts[qi] = kRedo;
+      } else {
+        TopK<O, MAXK> res;
+        int rc = 0;
+        v.finish(t, qw, bb, be, res, rc);
+#pragma unroll
+        for (int jj = 0; jj < MAXK; ++jj)
+          if (jj < k) {
+            ri[jj] = jj < rc ? res.it[jj] : 0xFFFFFFFFu;
+            rdd[jj] = jj < rc ? res.d[jj] : oinf<O>();
+          }
+        out_counts[qi] = uint32_t(rc);
+      }
+      cnt.rescored += v.rescored;
+    }
+  }
+  flush_counters(cnt, stats);
+}
+
+// The queries of k_forest_knn_s32 whose candidate list overflowed (count kRedo), answered
+// by the plain collector on their tree: blocks stride over the work items, each lane over
+// the item's queries; global stems only.
+template <class A, int K, int MAXK>
+__global__ __launch_bounds__(kBlock) void k_forest_knn_redo(
+    ForestView<A, K> f, const WorkItem* __restrict__ work, uint32_t n_work, const A* __restrict__ queries,
+    const uint32_t* __restrict__ order, int k, typename Ax<A>::O radius, int excl,
+    uint32_t* __restrict__ out_items, typename Ax<A>::O* __restrict__ out_d, uint32_t* __restrict__ out_counts,
+    unsigned long long* __restrict__ stats) {
+  using O = typename Ax<A>::O;
+  Counters cnt;
+  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const WorkItem wi = work[w];
+    for (uint32_t j = threadIdx.x; j < wi.count; j += kBlock) {
+      const uint64_t pos = uint64_t(wi.begin) + j;
+      const uint64_t qi = order ? order[pos] : pos;
+      if (out_counts[qi] != kRedo) continue;
+      const DevTree<A, K> t = f.tree(wi.tree);
       Small<A, K> q;
       Small<O, K> qw;
       load_query<A, K>(queries, qi, q, qw);
@@ -249,6 +347,7 @@ struct kd_forest {
   void* coords = nullptr;
   uint64_t plane = 0;
   uint32_t* items = nullptr;
+  float4* shadow = nullptr;  // FP32 shadow of the pooled entries (KD_TREE_SHADOW_F32)
   kd::dev::ForestTree* trees = nullptr;
   std::vector<kd::dev::ForestTree> h_trees;
   std::vector<double> h_bbox;  // per tree lo[4], scale[4]
@@ -289,7 +388,7 @@ void free_forest(kd_forest* f) {
   if (f->gstream) (void)hipStreamDestroy(f->gstream);
   for (hipEvent_t e : {f->ev_in, f->ev_out, f->ev_start, f->ev_stop})
     if (e) (void)hipEventDestroy(e);
-  for (void* p : {f->stems, (void*)f->leaf_start, f->coords, (void*)f->items, (void*)f->trees, (void*)f->bbox,
+  for (void* p : {f->stems, (void*)f->leaf_start, f->coords, (void*)f->items, (void*)f->shadow, (void*)f->trees, (void*)f->bbox,
                   (void*)f->counters, (void*)f->queue, (void*)f->work, (void*)f->order})
     if (p) (void)hipFree(p);
   delete f;
@@ -323,7 +422,7 @@ template <class F> void par_for(uint64_t count, F&& f) {
 }
 
 template <class A, int K>
-int forest_create_typed(const kd_tree_desc* descs, uint32_t ntrees, kd_forest** out) {
+int forest_create_typed(const kd_tree_desc* descs, uint32_t ntrees, bool with_shadow, kd_forest** out) {
   kd_forest* f = new kd_forest;
   f->dtype = descs[0].dtype;
   f->dims = K;
@@ -444,6 +543,24 @@ int forest_create_typed(const kd_tree_desc* descs, uint32_t ntrees, kd_forest** 
   alloc_up((void**)&f->leaf_start, leaf_start.data(), leaf_start.size() * 4);
   alloc_up(&f->coords, coords.data(), coords.size() * sizeof(A));
   alloc_up((void**)&f->items, items.data(), items.size() * 4);
+  if (with_shadow) {
+    // {x, y, z, item bits} for K <= 3, {x, y, z, w} for K = 4, zeroed padding past the end
+    std::vector<float4> sh(n + dev::kShadowPad, float4{0.0f, 0.0f, 0.0f, 0.0f});
+    par_for(ntrees, [&](uint64_t i) {
+      for (uint64_t e = pt_off[i]; e < pt_off[i + 1]; ++e) {
+        float4 v;
+        v.x = float(coords[0 * plane + e]);
+        v.y = float(coords[1 * plane + e]);
+        v.z = K > 2 ? float(coords[2 * plane + e]) : 0.0f;
+        if (K > 3)
+          v.w = float(coords[3 * plane + e]);
+        else
+          std::memcpy(&v.w, &items[e], 4);
+        sh[e] = v;
+      }
+    });
+    alloc_up((void**)&f->shadow, sh.data(), sh.size() * sizeof(float4));
+  }
   alloc_up((void**)&f->trees, f->h_trees.data(), f->h_trees.size() * sizeof(dev::ForestTree));
   alloc_up((void**)&f->bbox, f->h_bbox.data(), f->h_bbox.size() * sizeof(double));
   alloc_up((void**)&f->counters, nullptr, 4 * kd::dev::kStatSlots * sizeof(unsigned long long));
@@ -462,6 +579,7 @@ template <class A, int K> dev::ForestView<A, K> view_of(const kd_forest* f) {
   v.leaf_start = f->leaf_start;
   v.coords = static_cast<const A*>(f->coords);
   v.items = f->items;
+  v.shadow = f->shadow;
   v.plane = f->plane;
   v.trees = f->trees;
   return v;
@@ -515,6 +633,31 @@ int launch_forest_knn(kd_forest* f, const A* q, int k, double radius, uint32_t f
       default: return set_error(KD_ERR_INVALID, "bad group size %d", g);
     }
 #undef KD_FOREST_GROUP
+  } else if ((flags & KD_SCREEN_F32) && MAXK <= 16) {
+    if constexpr (std::is_same<A, float>::value) {
+      // the FP32 collector: its candidate list is static LDS, the stems take the rest of 40 KiB
+      constexpr int MK = MAXK <= 16 ? MAXK : 16;
+      constexpr size_t kList = size_t(dev::Screen32Vis<MK, A>::kCand) * 4 * dev::kBlock;
+      uint32_t cap = lds_cap;
+      while (cap && kList + size_t(cap) * sizeof(A) > 40960) cap >>= 1;
+      const size_t lds_s = size_t(cap) * sizeof(A);
+#define KD_FOREST_S32(W)                                                                              \
+  {                                                                                                   \
+    auto kern = dev::k_forest_knn_s32<A, K, MK, W>;                                                  \
+    hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, lds_s)), dim3(dev::kBlock), lds_s, s, view_of<A, K>(f), \
+                       f->work, f->n_work, f->queue, q, order, k, O(radius), excl, cap, items,         \
+                       static_cast<O*>(dists), counts, f->counters);                                  \
+  }
+      if (f->waves == 4) KD_FOREST_S32(4)
+      else if (f->waves == 5) KD_FOREST_S32(5)
+      else KD_FOREST_S32(3)
+#undef KD_FOREST_S32
+      KD_HIP(hipGetLastError());
+      const uint32_t rg = std::min<uint32_t>(std::max<uint32_t>(f->n_work, 1u), 2048u);
+      hipLaunchKernelGGL((dev::k_forest_knn_redo<A, K, MK>), dim3(rg), dim3(dev::kBlock), 0, s, view_of<A, K>(f),
+                         f->work, f->n_work, q, order, k, O(radius), excl, items, static_cast<O*>(dists), counts,
+                         f->counters);
+    }
   } else {
 #define KD_FOREST_THREAD(W)                                                                           \
   {                                                                                                   \
@@ -571,9 +714,13 @@ extern "C" {
 
 int kd_forest_create(const kd_tree_desc* descs, uint32_t ntrees, uint32_t tree_flags, kd_forest** out) {
   if (!descs || !out || ntrees == 0) return kd::set_error(KD_ERR_INVALID, "null argument or no trees");
-  if (tree_flags != 0) return kd::set_error(KD_ERR_UNSUPPORTED, "forest trees take no flags");
+  if (tree_flags & ~uint32_t(KD_TREE_SHADOW_F32)) return kd::set_error(KD_ERR_UNSUPPORTED, "unknown forest tree flags");
+  const bool with_shadow = (tree_flags & KD_TREE_SHADOW_F32) != 0;
+  if (with_shadow && descs[0].dtype != KD_F32)
+    return kd::set_error(KD_ERR_UNSUPPORTED, "forests keep an FP32 shadow for f32 trees only");
   try {
-    KD_DISPATCH_AK(descs[0].dtype, descs[0].dims, return (kd::forest_create_typed<A, KK>(descs, ntrees, out)));
+    KD_DISPATCH_AK(descs[0].dtype, descs[0].dims,
+                   return (kd::forest_create_typed<A, KK>(descs, ntrees, with_shadow, out)));
   } catch (const std::bad_alloc&) {
     return kd::set_error(KD_ERR_NO_MEMORY, "host allocation failed while uploading the forest");
   }
@@ -705,6 +852,8 @@ int kd_forest_nearest_n(kd_forest* f, const void* queries, uint32_t k, double ra
   if (f->q_off.empty()) return kd::set_error(KD_ERR_INVALID, "call kd_forest_plan first");
   if (f->m == 0) return KD_OK;
   if (!queries || !items || !dists || !counts) return kd::set_error(KD_ERR_INVALID, "null buffer");
+  if ((flags & KD_SCREEN_F32) && !f->shadow)
+    return kd::set_error(KD_ERR_UNSUPPORTED, "KD_SCREEN_F32 needs a forest created with KD_TREE_SHADOW_F32");
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (!(flags & KD_GRAPH)) {
     kd::ForestTimer ft(f, s);
@@ -907,7 +1056,9 @@ int kd_forest_autotune_bucket(int32_t dtype, int32_t dims, uint32_t ntrees, cons
     std::vector<kd_tree_desc> descs(ntrees);
     for (uint32_t i = 0; i < ntrees && rc == KD_OK; ++i) rc = kd_layout_desc(lays[i], &descs[i]);
     kd_forest* f = nullptr;
-    if (rc == KD_OK) rc = kd_forest_create(descs.data(), ntrees, 0, &f);
+    // the collector's candidates need the FP32 shadow
+    const uint32_t tflags = ((flags & KD_SCREEN_F32) && dtype == KD_F32) ? uint32_t(KD_TREE_SHADOW_F32) : 0u;
+    if (rc == KD_OK) rc = kd_forest_create(descs.data(), ntrees, tflags, &f);
     for (kd_layout* l : lays)
       if (l) kd_layout_free(l);
     if (rc) break;
