@@ -385,10 +385,10 @@ def main():
 def bench_forest(args, torch, ka):
     """Batched form: `--trees` independent f32 trees, sizes log-uniform in [1000, 262144]
     (seeded noisy unit-sphere clouds, radius jittered by N(0, 0.01)), each with a query set
-    of log-uniform size in [256, 65536] from the same law; nearest_n(k=8) (the plain
-    collector; the FP32 collector, KD_SCREEN_F32 on the forest's FP32 shadow, timed beside
-    it and checked equal) and within(r=0.02, passed as r^2 = 4e-4), one persistent launch per
-    batch, replayed from a hipGraph."""
+    of log-uniform size in [256, 65536] from the same law; nearest_n(k=8) with the FP32
+    collector (KD_SCREEN_F32 on the forest's FP32 shadow; the plain kernel timed beside it,
+    results checked equal) and within(r=0.02, passed as r^2 = 4e-4), one persistent launch
+    per batch, replayed from a hipGraph."""
     from concurrent.futures import ThreadPoolExecutor
     from kiddo_amd.datasets import noisy_sphere
 
@@ -410,7 +410,7 @@ def bench_forest(args, torch, ka):
             soffs[1:] = np.cumsum([nq[i] for i in smp])
             sq = torch.from_numpy(np.concatenate([qs[i] for i in smp])).cuda()
             bucket, bucket_ms = ka.Forest.autotune_bucket([pts[i] for i in smp], sq, soffs, 8,
-                                                          buckets=(16, 32, 64, 128))
+                                                          flags=ka.KD_SCREEN_F32, buckets=(16, 32, 64, 128))
             del sq
             log(f"bucket autotuned on {len(smp)} trees in {time.time() - t0:.1f}s: {bucket} {bucket_ms}")
         t0 = time.time()
@@ -442,15 +442,18 @@ def bench_forest(args, torch, ka):
         torch.cuda.synchronize()
         return time.perf_counter() - t0, kms, forest.query_stats(), chosen, out
 
-    dtc, kmsc, _, chosen_coll, outc = timed_knn(ka.KD_SCREEN_F32)
-    log(f"forest knn FP32 collector: {1e3 * dtc / args.steps:.1f} ms/step, {m * args.steps / dtc / 1e6:.1f} Mq/s, "
-        f"launch {chosen_coll}")
-    dt, kms, st, chosen, outp = timed_knn(0)
+    dtp, kmsp, _, chosen_plain, outp = timed_knn(0)
+    log(f"forest knn plain: {1e3 * dtp / args.steps:.1f} ms/step, {m * args.steps / dtp / 1e6:.1f} Mq/s, "
+        f"launch {chosen_plain}")
+    dt, kms, st, chosen, outc = timed_knn(ka.KD_SCREEN_F32)
     same = bool(torch.equal(outp.items, outc.items) and torch.equal(outp.distances, outc.distances)
                 and torch.equal(outp.counts, outc.counts))
     del outp, outc
-    log(f"forest knn: {1e3 * dt / args.steps:.1f} ms/step, {m * args.steps / dt / 1e6:.1f} Mq/s, "
-        f"launch {chosen}, FP32 collector equal {same}, {st}")
+    log(f"forest knn FP32 collector: {1e3 * dt / args.steps:.1f} ms/step, {m * args.steps / dt / 1e6:.1f} Mq/s, "
+        f"launch {chosen}, equal to plain {same}, {st}")
+    if not same:
+        log("collector results differ from the plain kernel's")
+        sys.exit(3)
     if not same:
         log("collector results differ from the plain kernel's")
         sys.exit(3)
@@ -487,7 +490,7 @@ def bench_forest(args, torch, ka):
     ftraffic = None
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            ftraffic = json.load(fh).get("k_forest_knn", {}).get("bytes_per_launch")
+            ftraffic = json.load(fh).get("k_forest_knn_s32", {}).get("bytes_per_launch")
     print(json.dumps({
         "metric": "queries/s, 4096-tree batched kNN k=8 (f32 3-D, persistent hipGraph launch)",
         "value": round(m * args.steps / dt, 1), "unit": "queries/s", "n_gpus": 1,
@@ -496,15 +499,14 @@ def bench_forest(args, torch, ka):
         "data": "synthetic (seeded noisy unit-sphere clouds)",
         "config": {"workload": "forest-4096-f32-knn8", "trees": T, "points": int(sizes.sum()),
                    "queries": m, "k": k, "launch": chosen, "bucket": bucket,
-                   "bucket_autotune_ms": bucket_ms},
-        "fp32_collector_qps": round(m * args.steps / dtc, 1),
-        "fp32_collector_kernel_ms": round(float(np.mean(kmsc)), 3), "fp32_collector_launch": chosen_coll,
-        "fp32_collector_equal": same,
+                   "bucket_autotune_ms": bucket_ms, "variant": "FP32 collector"},
+        "plain_qps": round(m * args.steps / dtp, 1), "plain_kernel_ms": round(float(np.mean(kmsp)), 3),
+        "plain_launch": chosen_plain, "collector_over_plain": round(dtp / dt, 3), "equal_to_plain": same,
         "within_r0.02": {"ms": round(wdt * 1e3, 2), "results": nres,
                          "results_per_s": round(nres / wdt, 1)},
         "roofline": {"bound": "hbm", "achieved": round(algo / kern_s / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(algo / kern_s / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": ftraffic, "kernel": "k_forest_knn", "kernel_ms": round(kern_s * 1e3, 3)},
+                     "traffic": ftraffic, "kernel": "k_forest_knn_s32", "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }), flush=True)
 

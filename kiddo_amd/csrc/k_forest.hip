@@ -166,6 +166,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
       O* rdd = out_d + qi * uint64_t(k);
       if (v.ovf) {
         out_counts[qi] = kRedo;
+        atomicAdd(stats + kRedoWord, 1ull);
       } else {
         TopK<O, MAXK> res;
         int rc = 0;
@@ -186,7 +187,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
 
 // The queries of k_forest_knn_s32 whose candidate list overflowed (count kRedo), answered
 // by the plain collector on their tree: blocks stride over the work items, each lane over
-// the item's queries; global stems only.
+// the item's queries; global stems only. The collector counts its overflows in the
+// forest's counters (cleared by k_forest_reset before it, on the same stream): with none,
+// the pass returns at once instead of reading every query's count (one forest runs one
+// launch at a time: its work queue is shared as well).
 template <class A, int K, int MAXK>
 __global__ __launch_bounds__(kBlock) void k_forest_knn_redo(
     ForestView<A, K> f, const WorkItem* __restrict__ work, uint32_t n_work, const A* __restrict__ queries,
@@ -194,6 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_forest_knn_redo(
     uint32_t* __restrict__ out_items, typename Ax<A>::O* __restrict__ out_d, uint32_t* __restrict__ out_counts,
     unsigned long long* __restrict__ stats) {
   using O = typename Ax<A>::O;
+  if (stats[kRedoWord] == 0ull) return;
   Counters cnt;
   for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
     const WorkItem wi = work[w];
@@ -301,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_forest_within(
 
 __global__ void k_forest_reset(unsigned int* queue, unsigned long long* stats) {
   if (threadIdx.x == 0) *queue = 0u;
-  for (uint32_t i = threadIdx.x; i < 4u * kStatSlots; i += blockDim.x) stats[i] = 0ull;
+  for (uint32_t i = threadIdx.x; i < uint32_t(kCounterWords); i += blockDim.x) stats[i] = 0ull;
 }
 
 // Morton keys tagged with the query's tree: sorting (tree, key) pairs reorders queries only
@@ -563,7 +568,7 @@ int forest_create_typed(const kd_tree_desc* descs, uint32_t ntrees, bool with_sh
   }
   alloc_up((void**)&f->trees, f->h_trees.data(), f->h_trees.size() * sizeof(dev::ForestTree));
   alloc_up((void**)&f->bbox, f->h_bbox.data(), f->h_bbox.size() * sizeof(double));
-  alloc_up((void**)&f->counters, nullptr, 4 * kd::dev::kStatSlots * sizeof(unsigned long long));
+  alloc_up((void**)&f->counters, nullptr, kd::dev::kCounterWords * sizeof(unsigned long long));
   alloc_up((void**)&f->queue, nullptr, 64);
   if (rc) {
     free_forest(f);

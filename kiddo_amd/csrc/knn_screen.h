@@ -11,7 +11,7 @@ namespace dev {
 
 // Pre-scan of the FP32 collector: the query descends to its leaf (the traversal's rule: a
 // sentinel stem sends it left) and takes the smallest block of 2^j sibling leaves holding 4k
-// entries (64 at least from k = 8 up; leaves under one stem are contiguous), whose FP32
+// entries (on f64 trees 64 at least from k = 8 up; leaves under one stem are contiguous), whose FP32
 // distances of certified in-radius entries fill top[] (all lanes of a wave do this in step).
 // The block is then the traversal's first visit. `up` receives the block's height in levels,
 // [b, e) its entries. The tree must not be empty.
@@ -31,7 +31,10 @@ __device__ __forceinline__ void prescan_top(const DevTree<A, K>& t, const Small<
   while (level < mid) step(t.lds[idx]);
   while (level < D) step(t.stems[idx]);
   const uint32_t leaf = idx - (1u << D);
-  const uint32_t want = v.k >= 8 && 4 * v.k < 64 ? 64u : uint32_t(4 * v.k);
+  // f64 trees: 64 entries at least from k = 8 up (measured best at k = 8 and 16 on a 16.8M-point
+  // tree); f32 trees: 4k (nearest_n(8) over 4096 trees of 1k..262k points: 17.4 ms with 32
+  // entries, 26.1 ms with 64)
+  const uint32_t want = !Screen32Vis<MAXK, A>::kExact && v.k >= 8 && 4 * v.k < 64 ? 64u : uint32_t(4 * v.k);
   uint32_t b = 0, e = 0;
   int up = 0;
   for (;; ++up) {
