@@ -101,25 +101,33 @@ def test_nan_queries(ka, oracle_mod, cuda, dtype):
     q[5] = np.nan
     O = OracleEngine(oracle_mod)
     ot = O.build(pts, 32)
-    for flags in (0, ka.KD_MORTON):
-        _announce("nan queries", np.dtype(dtype).name, flags)
-        G = GpuEngine(ka, flags=flags)
+    # the automatic forms (group kernels on a batch this small), one thread per query, and
+    # the FP32 collector (shadow + KD_SCREEN_F32: a NaN screen distance passes the screen
+    # and the query ends in the exact redo pass)
+    for flags, shadow in ((0, False), (ka.KD_MORTON, False), (ka.KD_MORTON | ka.KD_SCREEN_F32, True)):
+        G = GpuEngine(ka, flags=flags, shadow=shadow)
         gt = G.build(pts, 32)
-        gi, gd = G.nearest_one(gt, q)
-        oi, od = O.nearest_one(ot, q)
-        assert_same_bits(gi, oi, "nearest_one items")
-        assert_same_bits(gd, od, "nearest_one dists")
-        for k, r in ((8, math.inf), (4, 0.01)):
-            _cmp(G.nearest_n(gt, q, k, r), O.nearest_n(ot, q, k, r), f"nearest_n({k}, {r})")
-        go, gi, gd = G.within(gt, q, 0.004)
-        oo, oi, od = O.within(ot, q, 0.004)
-        assert_same_bits(go, oo, "within offsets")
-        assert_same_bits(gi, oi, "within items")
-        _cmp(G.best_n_within(gt, q, 0.01, 5), O.best_n_within(ot, q, 0.01, 5), "best_n_within")
-        gi, gd = G.approx_nearest_one(gt, q)
-        oi, od = O.approx_nearest_one(ot, q)
-        assert_same_bits(gi, oi, "approx items")
-        assert_same_bits(gd, od, "approx dists")
+        for variant in (0, 1):
+            _announce("nan queries", np.dtype(dtype).name, flags, variant)
+            gt.tune(variant, 0, -1)
+            gt.tune_best_n_within(variant)
+            gi, gd = G.nearest_one(gt, q)
+            oi, od = O.nearest_one(ot, q)
+            assert_same_bits(gi, oi, "nearest_one items")
+            assert_same_bits(gd, od, "nearest_one dists")
+            for k, r in ((8, math.inf), (4, 0.01), (16, math.inf)):
+                _cmp(G.nearest_n(gt, q, k, r), O.nearest_n(ot, q, k, r), f"nearest_n({k}, {r}) variant={variant}")
+            go, gi, gd = G.within(gt, q, 0.004)
+            oo, oi, od = O.within(ot, q, 0.004)
+            assert_same_bits(go, oo, "within offsets")
+            assert_same_bits(gi, oi, "within items")
+            _cmp(G.best_n_within(gt, q, 0.01, 5), O.best_n_within(ot, q, 0.01, 5), "best_n_within")
+            gi, gd = G.approx_nearest_one(gt, q)
+            oi, od = O.approx_nearest_one(ot, q)
+            assert_same_bits(gi, oi, "approx items")
+            assert_same_bits(gd, od, "approx dists")
+        gt.tune(0, 0, -1)
+        gt.tune_best_n_within(0)
     nan_rows = np.isnan(q).any(axis=1)
     assert np.all(oi[nan_rows] == 0)
 
