@@ -58,34 +58,37 @@ __global__ __launch_bounds__(kBlock) void k_periodic_nearest_one(DevTree<A, K> t
                                                                  unsigned long long* __restrict__ stats) {
   KD_QUERY_PROLOGUE
   if (active) {
-    NearestOneVis<O> home{oinf<O>(), 0u};
-    traverse(t, q, wrap_qw(qw, met), home, cnt);
-    O best = home.best;
-    uint32_t item = home.item;
-    // per axis and shift: the shifted coordinate and dist1 of its offset from [0, L]
-    Small<A, K> wc[3];
-    Small<O, K> sq[3];
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      wc[s].fill(A(0));
-      sq[s].fill(O(0));
-#pragma unroll
-      for (int a = 0; a < K; ++a) {
-        const A L = box.v[a], x = q[a];
-        const A w = s == 0 ? A(x - L) : s == 1 ? x : A(x + L);
-        const A o = w < A(0) ? A(fabs(w - A(0))) : (w > L ? A(fabs(w - L)) : A(0));
-        wc[s].set(a, w);
-        sq[s].set(a, met.d1(O(o)));
-      }
-    }
+    // one traversal call site for the home query and every image (a second inlined DFS and
+    // the image tables held across it took 127 VGPRs); the tables are recomputed from the
+    // query and the box (kernel arguments) at each choice of the next image
+    auto shifted = [&](int s, int a) -> A {
+      const A L = box.v[a], x = q[a];
+      return s == 0 ? A(x - L) : s == 1 ? x : A(x + L);
+    };
+    auto off1 = [&](int s, int a) -> O {
+      const A L = box.v[a], w = shifted(s, a);
+      const A o = w < A(0) ? A(fabs(w - A(0))) : (w > L ? A(fabs(w - L)) : A(0));
+      return met.d1(O(o));
+    };
     constexpr int kImages = K == 2 ? 9 : K == 3 ? 27 : 81;
     constexpr int kHome = kImages / 2;  // all shifts zero
+    O best = oinf<O>();
+    uint32_t item = 0u;
     O prev_lb = O(0);
     int prev = -1;
+    Small<A, K> wq = q;
+    Small<O, K> wqw = qw;
     for (;;) {
+      NearestOneVis<O> img{oinf<O>(), 0u};
+      traverse(t, wq, wrap_qw(wqw, met), img, cnt);
+      if (img.best < best || (img.best == best && img.item < item)) {
+        best = img.best;
+        item = img.item;
+      }
       // the next image after (prev_lb, prev) in (lower bound, index) order
       O nlb = oinf<O>();
       int next = -1;
+#pragma unroll 1
       for (int i = 0; i < kImages; ++i) {
         if (i == kHome) continue;
         int r = i;
@@ -97,10 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_periodic_nearest_one(DevTree<A, K> t
         }
         O lb = O(0);
 #pragma unroll
-        for (int a = 0; a < K; ++a) {
-          const O s0 = sq[0][a], s1 = sq[1][a], s2 = sq[2][a];
-          lb = met.combine(lb, dig[a] == 0 ? s0 : dig[a] == 1 ? s1 : s2);
-        }
+        for (int a = 0; a < K; ++a) lb = met.combine(lb, off1(dig[a], a));
         const bool after = prev < 0 || lb > prev_lb || (lb == prev_lb && i > prev);
         const bool before = lb < nlb || (lb == nlb && (next < 0 || i < next));
         if (after && before && !(lb > best)) {
@@ -109,24 +109,13 @@ __global__ __launch_bounds__(kBlock) void k_periodic_nearest_one(DevTree<A, K> t
         }
       }
       if (next < 0) break;
-      Small<A, K> wq;
-      Small<O, K> wqw;
-      wq.fill(A(0));
-      wqw.fill(O(0));
       int r = next;
 #pragma unroll
       for (int a = K - 1; a >= 0; --a) {
-        const int dg = r % 3;
+        const A w = shifted(r % 3, a);
         r /= 3;
-        const A w = dg == 0 ? wc[0][a] : dg == 1 ? wc[1][a] : wc[2][a];
         wq.set(a, w);
         wqw.set(a, O(w));
-      }
-      NearestOneVis<O> img{oinf<O>(), 0u};
-      traverse(t, wq, wrap_qw(wqw, met), img, cnt);
-      if (img.best < best || (img.best == best && img.item < item)) {
-        best = img.best;
-        item = img.item;
       }
       prev_lb = nlb;
       prev = next;
