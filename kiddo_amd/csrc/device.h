@@ -1493,6 +1493,15 @@ template <int MAXK, class A = double> struct Screen32Vis {
     }
     refresh();
   }
+  // an entry's distance in the tree's own arithmetic: on f32 trees the shadow's (the same
+  // operations in the same order; one 16-byte load instead of one per coordinate plane)
+  template <int K, class QW>
+  __device__ __forceinline__ O final_dist(const DevTree<A, K>& t, uint32_t p, const QW& qw) const {
+    if constexpr (kExact)
+      return shadow_sq<K>(t.shadow[p], qf);
+    else
+      return point_dist(t, p, qw);
+  }
   // The answer: the pre-scan block's entries [bb, be) and the listed candidates that pass
   // the final threshold, measured in FP64, in (distance, item) order; ok entries are those
   // within the radius.
@@ -1541,7 +1550,7 @@ template <int MAXK, class A = double> struct Screen32Vis {
     int nok = 0;
 #pragma unroll
     for (int j = 0; j < MAXK; ++j) {
-      const O dd = point_dist(t, pidx[j], qw);
+      const O dd = final_dist(t, pidx[j], qw);
       const uint32_t it = t.items[pidx[j]];
       const bool ok = (uint32_t(j) < taken) & ((dd < radius) | (!excl & (dd == radius)));
       // the item is masked rather than selected (a select of a loaded value becomes a
@@ -1556,7 +1565,7 @@ template <int MAXK, class A = double> struct Screen32Vis {
     uint32_t p = bb;
     while (next_cand(p)) {
       ++taken;
-      const O dd = point_dist(t, p, qw);
+      const O dd = final_dist(t, p, qw);
       const uint32_t it = t.items[p];
       const bool ok = (dd < radius) | (!excl & (dd == radius));
       nok += ok ? 1 : 0;
