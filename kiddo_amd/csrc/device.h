@@ -1493,14 +1493,20 @@ template <int MAXK, class A = double> struct Screen32Vis {
     }
     refresh();
   }
-  // an entry's distance in the tree's own arithmetic: on f32 trees the shadow's (the same
-  // operations in the same order; one 16-byte load instead of one per coordinate plane)
+  // an entry's distance in the tree's own arithmetic and its item: on f32 trees both from the
+  // shadow entry (the same operations in the same order; the item bits in w for K <= 3), one
+  // 16-byte load instead of one per coordinate plane and one for the item
   template <int K, class QW>
-  __device__ __forceinline__ O final_dist(const DevTree<A, K>& t, uint32_t p, const QW& qw) const {
-    if constexpr (kExact)
-      return shadow_sq<K>(t.shadow[p], qf);
-    else
-      return point_dist(t, p, qw);
+  __device__ __forceinline__ void final_entry(const DevTree<A, K>& t, uint32_t p, const QW& qw, O& dd,
+                                              uint32_t& it) const {
+    if constexpr (kExact) {
+      const float4 sh = t.shadow[p];
+      dd = shadow_sq<K>(sh, qf);
+      it = K <= 3 ? __float_as_uint(sh.w) : t.items[p];  // K = 4: w is a coordinate
+    } else {
+      dd = point_dist(t, p, qw);
+      it = t.items[p];
+    }
   }
   // The answer: the pre-scan block's entries [bb, be) and the listed candidates that pass
   // the final threshold, measured in FP64, in (distance, item) order; ok entries are those
@@ -1550,8 +1556,9 @@ template <int MAXK, class A = double> struct Screen32Vis {
     int nok = 0;
 #pragma unroll
     for (int j = 0; j < MAXK; ++j) {
-      const O dd = final_dist(t, pidx[j], qw);
-      const uint32_t it = t.items[pidx[j]];
+      O dd;
+      uint32_t it;
+      final_entry(t, pidx[j], qw, dd, it);
       const bool ok = (uint32_t(j) < taken) & ((dd < radius) | (!excl & (dd == radius)));
       // the item is masked rather than selected (a select of a loaded value becomes a
       // branch around its load); the empty asm keeps the mask opaque
@@ -1565,8 +1572,9 @@ template <int MAXK, class A = double> struct Screen32Vis {
     uint32_t p = bb;
     while (next_cand(p)) {
       ++taken;
-      const O dd = final_dist(t, p, qw);
-      const uint32_t it = t.items[p];
+      O dd;
+      uint32_t it;
+      final_entry(t, p, qw, dd, it);
       const bool ok = (dd < radius) | (!excl & (dd == radius));
       nok += ok ? 1 : 0;
       res.insert_if(ok && TopK<O, MAXK>::less(dd, it, res.d[MAXK - 1], res.it[MAXK - 1]), dd, it);
