@@ -853,7 +853,9 @@ def bench_mixture(args, torch, ka):
                                                      "frac": round(fill_bytes / (fill_kernel_ms / 1e3) / 1e9
                                                                    / HBM_PEAK_GBS, 4),
                                                      "algo_bytes": int(fill_bytes),
-                                                     "traffic": tr.get("k_within_fill_group", {}).get("bytes_per_pass"),
+                                                     "traffic": tr.get("k_within_fill_group_4lanes"
+                                                                       if wshape.get("fill_group") == 4
+                                                                       else "k_within_fill_group", {}).get("bytes_per_pass"),
                                                      "kernel": "k_within_fill_group"},
                                    "roofline_count": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                                                       "achieved": round(count_bytes / (count_kernel_ms / 1e3) / 1e9, 1),
