@@ -1799,11 +1799,12 @@ template <class O, int G> struct GroupWithinVis {
 // finish(); lanes of one group are in one wave, so a wave barrier orders the ring's stores
 // before its loads.
 template <class O, int G> struct GroupWithinStageVis {
-  // 4 lanes: a ring of 32 and segments of 16 results (16 KiB of LDS per block, 8 blocks per
-  // CU); more lanes: 64 and 32 (the flush leaves fewer than kAlign pending and comes when
-  // more than kRing - G are, so at most kRing are ever pending)
-  static constexpr uint32_t kRing = G <= 4 ? 32 : 64;
-  static constexpr uint32_t kAlign = G <= 4 ? 16 : 32;
+  // (the flush leaves fewer than kAlign pending and comes when more than kRing - G are, so
+  // at most kRing are ever pending). A ring of 32 in segments of 16 at 4 lanes (16 KiB of LDS
+  // per block) filled the 10M-point mixture as fast as 8 lanes but read 3.4x the HBM bytes
+  // (65.7 against 19.3 GB), and the autotuner, which prefers fewer lanes, then took it.
+  static constexpr uint32_t kRing = 64;
+  static constexpr uint32_t kAlign = 32;
   static_assert(kRing - G >= kAlign && kRing % kAlign == 0, "ring and segment sizes");
   O radius;
   bool excl;
