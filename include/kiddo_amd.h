@@ -321,8 +321,8 @@ int kd_last_kernel_ms(kd_tree* tree, float* ms);
  * 0, or KD_TREE_SHADOW_F32 for f32 trees (an FP32 shadow of the pooled entries, 16 B each),
  * which kd_forest_nearest_n's KD_SCREEN_F32 needs (the FP32 collector for k <= 16, thread
  * per query, exact redo pass for overflowed candidate lists; identical results; measured on
- * 4096 trees of 1k..262k points: 17.2 against 24.1 ms at k = 8, 30.3 against 41.4 ms at
- * k = 16). */
+ * 4096 trees of 1k..262k points: 15.9 against 24.0 ms at k = 8, 30.3 against 41.4 ms at
+ * k = 16 before the end phase read distances and items from the shadow). */
 typedef struct kd_forest kd_forest;
 int kd_forest_create(const kd_tree_desc* descs, uint32_t ntrees, uint32_t tree_flags, kd_forest** out);
 int kd_forest_destroy(kd_forest* forest);
